@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Throughput of the per-particle timestep on one MI355X.
+
+Workload (the 10M-particle production shape): 10,000,000 particles from 16
+point sources, 360 x 181 x 60 met grid (1 deg global) with time levels 6 h
+apart, 600 s step, Hanna + Langevin with 4 vertical sub-steps and PBL
+reflection, dry deposition v_d = 2e-3 m/s, concentration grid 720 x 361 x 5
+accumulated every step and cleared every simulated hour, Morton re-sort every
+``--sort-every`` steps, met brackets switched on device as simulated time
+crosses 6 h boundaries.  Inputs are synthetic and resident in HBM before the
+timed region; a step is one pass of the hot path over all particles.
+
+Prints one JSON line: particle-steps/s (whole job), the step kernel's
+roofline figure from HIP events on the library's stream, and a CPU baseline
+(the oracle, OpenMP, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "flexpart-gpu_amd")
+
+METRIC = "particle-steps/s at 10M (1 GPU) + adjoint/forward time at rel-err ≤1e-3"
+BYTES_PER_PARTICLE_STEP = 100  # 44 B state read + 56 B state and velocity written
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_global_10m_traffic.md")
+
+
+def _package():
+    name = "flexpart_gpu_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(
+        name, os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _traffic_bytes():
+    """HBM bytes per step-kernel launch measured with rocprofv3 --pmc (or None)."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(w, threads: int, sample_every: int = 5, steps: int = 4):
+    """The oracle on every ``sample_every``-th particle for ``steps`` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    parts = w.particles[::sample_every].copy()
+    u0, v0, w0 = w.met.levels[0]
+    u1, v1, w1 = w.met.levels[1]
+    met = dict(shape=w.met.shape, u0=u0, v0=v0, w0=w0, u1=u1, v1=v1, w1=w1,
+               level_heights=w.met.level_heights)
+    pbl = dict(shape=w.pbl.shape, ustar=w.pbl.ustar, wstar=w.pbl.wstar, hmix=w.pbl.hmix,
+               oli=w.pbl.oli)
+    counter = (0, 0, 0, 0)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        counter = oracle.step(parts, met, pbl, dict(
+            dt_seconds=w.dt_seconds, alpha=k * w.dt_seconds / 21600.0, x_scale=w.x_scale,
+            y_scale=w.y_scale, key=w.key, counter=counter, dry_mode=1, dry_vd=2e-3,
+            n_threads=threads))
+    dt = time.perf_counter() - t0
+    return {"value": len(parts) * steps / dt, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(parts)} particles (every {sample_every}th of the 10M workload) x "
+                      f"{steps} steps, C oracle with OpenMP, {dt:.2f} s wall"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=144, help="timed steps (144 = the 24 h run)")
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--sort-every", type=int, default=12)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--ppt", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: the timestep shards by particle with no "
+              "exchange inside a step, so only --gpus 1 is supported in this build",
+              file=sys.stderr, flush=True)
+        return 2
+
+    import numpy as np
+    import torch
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    n = args.particles
+    t_gen = time.perf_counter()
+    w = W.global_10m(n)
+    print(f"# workload generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    levels = [[torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in lvl] for lvl in w.met.levels]
+    torch.cuda.synchronize()
+
+    ctx = fp.GpuTimestep(n, w.met.shape, n_species=1, out_shape=w.out_shape, device=0)
+    ctx.set_launch(args.block, args.ppt)
+    ctx.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli)
+    ctx.configure_grid(w.outheights, w.refine, 0, w.mass_scale)
+    steps_per_hour = int(round(3600.0 / w.dt_seconds))
+    sec_per_level = w.met.seconds_between_levels
+
+    def reset():
+        ctx.upload_store(w.particles)
+        ctx.clear_grid()
+
+    def run(n_steps):
+        bracket = -1
+        counter = (0, 0, 0, 0)
+        for k in range(n_steps):
+            t = k * w.dt_seconds
+            b, alpha = w.alpha(t)
+            if b != bracket:  # upload_dual_wind_if_bracket_changed (timeloop.rs:1378-1402)
+                ptrs = [levels[b][c].data_ptr() for c in range(3)] + \
+                       [levels[b + 1][c].data_ptr() for c in range(3)]
+                ctx.upload_met_bracket_dev(ptrs, w.met.level_heights)
+                bracket = b
+            if args.sort_every and k % args.sort_every == 0:
+                ctx.sort_morton(10, want_map=False)
+            if k % steps_per_hour == 0:
+                ctx.clear_grid()
+            cfg = fp.StepConfig(dt_seconds=w.dt_seconds, alpha=alpha, x_scale=w.x_scale,
+                                y_scale=w.y_scale, key=w.key, counter=counter, dry_mode=1,
+                                dry_vd=2e-3, grid_fused=True)
+            counter = ctx.step(cfg)
+
+    reset()
+    run(args.warmup)
+    ctx.sync()
+    reset()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    run(args.steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    step_ms, launches = ctx.profile_read(fp.KERNEL_STEP)
+    sort_ms, sorts = ctx.profile_read(fp.KERNEL_SORT)
+    ctx.profile(False)
+
+    # sanity: the fused grid of the last hour counts every particle once per step
+    counts, _ = ctx.download_grid()
+    steps_in_last_hour = args.steps - ((args.steps - 1) // steps_per_hour) * steps_per_hour
+    if int(counts.astype(np.int64).sum()) != n * steps_in_last_hour:
+        raise RuntimeError("grid count check failed")
+    ctx.close()
+
+    avg_s = step_ms / 1e3 / max(launches, 1)
+    achieved = n * BYTES_PER_PARTICLE_STEP / avg_s / 1e9
+    traffic = _traffic_bytes()
+    result = {
+        "metric": METRIC,
+        "value": n * args.steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "global_10m: 10M particles from 16 point sources, 360x181x60 met "
+                        "(6 h levels), 600 s step, 4 vertical sub-steps, dry deposition, "
+                        "fused 720x361x5 gridding",
+            "particles": n,
+            "met_grid": list(w.met.shape),
+            "dt_seconds": w.dt_seconds,
+            "sort_every": args.sort_every,
+            "block": args.block,
+            "particles_per_thread": args.ppt,
+            "step_kernel_ms": avg_s * 1e3,
+            "sort_ms_per_call": (sort_ms / sorts) if sorts else None,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+        result["cpu_baseline"] = cpu_baseline(w, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

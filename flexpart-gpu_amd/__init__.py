@@ -1,0 +1,393 @@
+"""flexpart-gpu_amd: MI355X (gfx950) per-particle timestep behind a C ABI.
+
+This module is the Python host side of ``include/flexpart_hip.h``: a ctypes
+binding of ``lib/libflexpart_hip.so`` plus ``GpuTimestep``, a thin object that
+mirrors how the reference time loop drives its GPU backend
+(simulation/timeloop.rs:915-1316 over the ``crate::gpu`` API): particle
+uploads and downloads, met-bracket and PBL uploads, the fused step, the
+concentration grid, compaction and the Morton re-sort.
+
+There is no CPU fallback: importing works anywhere, but every call that needs
+the library raises ``LibraryMissing`` when it is not built and ``FpError``
+when the HIP runtime reports a failure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libflexpart_hip.so")
+
+FP_OK = 0
+FP_E_INVALID = -1
+FP_E_DEVICE = -2
+FP_E_NOMEM = -3
+FP_E_STATE = -4
+
+FLAG_ACTIVE = 1
+MAX_SPECIES = 4
+MAX_OUTPUT_LEVELS = 16
+
+DRY_OFF, DRY_UNIFORM, DRY_PER_SLOT = 0, 1, 2
+WET_OFF, WET_UNIFORM, WET_PER_SLOT, WET_PRECIP_FIELD = 0, 1, 2, 3
+LOSS_NONE, LOSS_MIN_MASS = 0, 1
+KERNEL_STEP, KERNEL_GRID, KERNEL_COMPACT, KERNEL_SORT = 0, 1, 2, 3
+
+# The reference's 96-byte Particle (particles/mod.rs:76-168) == fp_particle.
+PARTICLE_DTYPE = np.dtype(
+    [
+        ("pos_x", "<f4"), ("pos_y", "<f4"), ("pos_z", "<f4"),
+        ("cell_x", "<i4"), ("cell_y", "<i4"), ("flags", "<u4"),
+        ("mass", "<f4", (MAX_SPECIES,)),
+        ("vel_u", "<f4"), ("vel_v", "<f4"), ("vel_w", "<f4"),
+        ("turb_u", "<f4"), ("turb_v", "<f4"), ("turb_w", "<f4"),
+        ("time", "<i4"), ("timestep", "<i4"), ("time_mem", "<i4"), ("time_split", "<i4"),
+        ("release_point", "<i4"), ("class_id", "<i4"), ("cbt", "<i4"), ("pad0", "<u4"),
+    ]
+)
+assert PARTICLE_DTYPE.itemsize == 96
+
+SOA_FIELDS = ("pos_x", "pos_y", "pos_z", "cell_x", "cell_y", "flags", "mass0", "mass1",
+              "mass2", "mass3", "vel_u", "vel_v", "vel_w", "turb_u", "turb_v", "turb_w",
+              "time", "timestep", "time_mem", "time_split", "release_point", "class_id",
+              "cbt", "pad0")
+
+
+class LibraryMissing(RuntimeError):
+    """libflexpart_hip.so is not built (run ``python flexpart-gpu_amd/build.py``)."""
+
+
+class FpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class CtxDesc(C.Structure):
+    _fields_ = [("capacity", C.c_uint64), ("met_nx", C.c_uint32), ("met_ny", C.c_uint32),
+                ("met_nz", C.c_uint32), ("n_species", C.c_uint32), ("out_nx", C.c_uint32),
+                ("out_ny", C.c_uint32), ("out_nz", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [
+        ("dt_seconds", C.c_float), ("alpha", C.c_float),
+        ("x_scale", C.c_float), ("y_scale", C.c_float), ("z_scale", C.c_float),
+        ("direction", C.c_int32), ("key", C.c_uint32 * 2), ("counter", C.c_uint32 * 4),
+        ("n_substeps", C.c_uint32), ("min_height_m", C.c_float), ("rho_grad_over_rho", C.c_float),
+        ("dry_mode", C.c_uint32), ("dry_vd", C.c_float), ("dry_href", C.c_float),
+        ("wet_mode", C.c_uint32), ("wet_lambda", C.c_float), ("wet_fraction", C.c_float),
+        ("wet_a", C.c_float), ("wet_b", C.c_float),
+        ("loss_mode", C.c_uint32), ("loss_min_mass", C.c_float),
+        ("grid_fused", C.c_uint32), ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class GridParams(C.Structure):
+    _fields_ = [("refine_x", C.c_uint32), ("refine_y", C.c_uint32), ("species", C.c_uint32),
+                ("mass_scale", C.c_float), ("outheights", C.c_float * MAX_OUTPUT_LEVELS)]
+
+
+class SortParams(C.Structure):
+    _fields_ = [("bits_per_axis", C.c_uint32), ("explicit_bounds", C.c_uint32),
+                ("bounds", C.c_double * 6)]
+
+
+_f = C.POINTER(C.c_float)
+_u32 = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/flexpart_hip.h
+ABI = {
+    "fp_version": (C.c_char_p, []),
+    "fp_last_error": (C.c_char_p, [_vp]),
+    "fp_ctx_create": (C.c_int, [C.c_int, C.POINTER(CtxDesc), C.POINTER(_vp)]),
+    "fp_ctx_destroy": (C.c_int, [_vp]),
+    "fp_sync": (C.c_int, [_vp]),
+    "fp_particles_upload_slots": (C.c_int, [_vp, _u32, _vp, C.c_uint64]),
+    "fp_particles_upload": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fp_particles_download": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fp_particles_set_dispatch_count": (C.c_int, [_vp, C.c_uint64]),
+    "fp_particles_dispatch_count": (C.c_uint64, [_vp]),
+    "fp_particles_soa_pointers": (C.c_int, [_vp, C.POINTER(_vp), C.c_uint32]),
+    "fp_met_upload_bracket": (C.c_int, [_vp, _f, _f, _f, _f, _f, _f, _f]),
+    "fp_met_upload_bracket_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
+    "fp_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
+    "fp_forcing_upload": (C.c_int, [_vp, _f, _f, _f]),
+    "fp_step": (C.c_int, [_vp, C.POINTER(StepParams), _u32]),
+    "fp_grid_clear": (C.c_int, [_vp]),
+    "fp_grid_configure": (C.c_int, [_vp, C.POINTER(GridParams)]),
+    "fp_grid_accumulate": (C.c_int, [_vp]),
+    "fp_grid_download": (C.c_int, [_vp, _u32, _u32]),
+    "fp_compact": (C.c_int, [_vp, C.POINTER(C.c_uint64), _u32]),
+    "fp_sort_morton": (C.c_int, [_vp, C.POINTER(SortParams), _u32]),
+    "fp_philox_blocks": (C.c_int, [_vp, _u32, _u32, C.c_uint64, _u32]),
+    "fp_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "fp_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "fp_profile_reset": (C.c_int, [_vp]),
+    "fp_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+}
+
+_LIB: C.CDLL | None = None
+
+
+def library() -> C.CDLL:
+    """Load libflexpart_hip.so (raises LibraryMissing when it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} not found; run python flexpart-gpu_amd/build.py")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_f)
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+@dataclass
+class StepConfig:
+    """Per-step parameters (fp_step_params)."""
+    dt_seconds: float = 600.0
+    alpha: float = 0.0
+    x_scale: float = 1.0
+    y_scale: float = 1.0
+    z_scale: float = 1.0
+    direction: int = 1
+    key: tuple = (0xDECAFBAD, 0x12345678)
+    counter: tuple = (0, 0, 0, 0)
+    n_substeps: int = 4
+    min_height_m: float = 0.01
+    rho_grad_over_rho: float = 0.0
+    dry_mode: int = DRY_OFF
+    dry_vd: float = 0.0
+    dry_href: float = 15.0
+    wet_mode: int = WET_OFF
+    wet_lambda: float = 0.0
+    wet_fraction: float = 0.0
+    wet_a: float = 0.0
+    wet_b: float = 0.0
+    loss_mode: int = LOSS_NONE
+    loss_min_mass: float = 0.0
+    grid_fused: bool = False
+
+    def to_c(self) -> StepParams:
+        p = StepParams()
+        for name, _ in StepParams._fields_:
+            if name == "reserved":
+                continue
+            v = getattr(self, name)
+            if name in ("key", "counter"):
+                arr = getattr(p, name)
+                for i, x in enumerate(v):
+                    arr[i] = int(x) & 0xFFFFFFFF
+            elif name == "grid_fused":
+                p.grid_fused = 1 if v else 0
+            else:
+                setattr(p, name, v)
+        return p
+
+
+class GpuTimestep:
+    """One device context: particle store, met bracket, PBL, output grid.
+
+    Method names follow the reference's ``crate::gpu`` surface
+    (ParticleBuffers, DualWindBuffers, PblBuffers, the encode_* passes) so
+    that a caller reads like the reference time loop.
+    """
+
+    def __init__(self, capacity: int, met_shape, n_species: int = 4, out_shape=None,
+                 device: int = 0):
+        lib = library()
+        nx, ny, nz = met_shape
+        ox, oy, oz = out_shape if out_shape is not None else (0, 0, 0)
+        d = CtxDesc(capacity, nx, ny, nz, n_species, ox, oy, oz)
+        h = _vp()
+        rc = lib.fp_ctx_create(device, C.byref(d), C.byref(h))
+        if rc != FP_OK:
+            raise FpError(rc, lib.fp_last_error(None).decode())
+        self._lib = lib
+        self._h = h
+        self.capacity = int(capacity)
+        self.met_shape = (nx, ny, nz)
+        self.out_shape = (ox, oy, oz)
+        self.n_species = n_species
+
+    # -- plumbing
+    def _chk(self, rc):
+        if rc != FP_OK:
+            raise FpError(rc, self._lib.fp_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.fp_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._chk(self._lib.fp_sync(self._h))
+
+    # -- particles (ParticleBuffers)
+    def upload_store(self, particles: np.ndarray):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        self._chk(self._lib.fp_particles_upload(self._h, p.ctypes.data, len(p)))
+
+    def upload_particle_slots(self, slots, particles: np.ndarray):
+        s = np.ascontiguousarray(np.asarray(slots, dtype=np.uint32))
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        if len(s) != len(p):
+            raise ValueError("slots and particles differ in length")
+        self._chk(self._lib.fp_particles_upload_slots(self._h, s.ctypes.data_as(_u32),
+                                                      p.ctypes.data, len(p)))
+
+    def download_particles(self, n=None) -> np.ndarray:
+        n = self.capacity if n is None else int(n)
+        out = np.zeros(n, PARTICLE_DTYPE)
+        self._chk(self._lib.fp_particles_download(self._h, out.ctypes.data, n))
+        return out
+
+    def set_dispatch_count(self, n: int):
+        self._chk(self._lib.fp_particles_set_dispatch_count(self._h, int(n)))
+
+    def particle_count(self) -> int:
+        return int(self._lib.fp_particles_dispatch_count(self._h))
+
+    def soa_pointers(self) -> dict:
+        ptrs = (_vp * len(SOA_FIELDS))()
+        self._chk(self._lib.fp_particles_soa_pointers(self._h, ptrs, len(SOA_FIELDS)))
+        return dict(zip(SOA_FIELDS, [int(p or 0) for p in ptrs]))
+
+    # -- met / PBL / forcing
+    def upload_met_bracket(self, u0, v0, w0, u1, v1, w1, level_heights):
+        n = int(np.prod(self.met_shape))
+        arrs = [_f32(a, n) for a in (u0, v0, w0, u1, v1, w1)]
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_met_upload_bracket(self._h, *[_fp(a) for a in arrs], _fp(lh)))
+
+    def upload_met_bracket_dev(self, ptrs, level_heights):
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_met_upload_bracket_dev(self._h, *[_vp(int(p)) for p in ptrs],
+                                                      _fp(lh)))
+
+    def upload_pbl(self, ustar, wstar, hmix, oli, precip=None):
+        n = self.met_shape[0] * self.met_shape[1]
+        arrs = [_f32(a, n) for a in (ustar, wstar, hmix, oli)]
+        pr = None if precip is None else _f32(precip, n)
+        self._chk(self._lib.fp_pbl_upload(self._h, *[_fp(a) for a in arrs], _fp(pr)))
+
+    def upload_forcing(self, dry_vd=None, wet_lambda=None, wet_fraction=None):
+        conv = [None if a is None else _f32(a, self.capacity) for a in (dry_vd, wet_lambda, wet_fraction)]
+        self._chk(self._lib.fp_forcing_upload(self._h, *[_fp(a) for a in conv]))
+
+    # -- the step
+    def step(self, cfg: StepConfig) -> tuple:
+        p = cfg.to_c()
+        nxt = (C.c_uint32 * 4)()
+        self._chk(self._lib.fp_step(self._h, C.byref(p), nxt))
+        return tuple(int(v) for v in nxt)
+
+    # -- concentration grid
+    def configure_grid(self, outheights=None, refine=(1, 1), species=0, mass_scale=1.0e6):
+        g = GridParams()
+        g.refine_x, g.refine_y = refine
+        g.species = species
+        g.mass_scale = mass_scale
+        if outheights is not None:
+            for i, v in enumerate(outheights):
+                g.outheights[i] = float(v)
+        self._chk(self._lib.fp_grid_configure(self._h, C.byref(g)))
+
+    def clear_grid(self):
+        self._chk(self._lib.fp_grid_clear(self._h))
+
+    def accumulate_grid(self):
+        self._chk(self._lib.fp_grid_accumulate(self._h))
+
+    def download_grid(self):
+        n = int(np.prod(self.out_shape))
+        counts = np.zeros(n, np.uint32)
+        mass = np.zeros(n, np.uint32)
+        self._chk(self._lib.fp_grid_download(self._h, counts.ctypes.data_as(_u32),
+                                             mass.ctypes.data_as(_u32)))
+        return counts, mass
+
+    # -- compaction / sort
+    def compact(self, want_map: bool = True):
+        n = self.particle_count()
+        smap = np.zeros(max(n, 1), np.uint32) if want_map else None
+        a = C.c_uint64()
+        self._chk(self._lib.fp_compact(self._h, C.byref(a),
+                                       None if smap is None else smap.ctypes.data_as(_u32)))
+        return int(a.value), (None if smap is None else smap[: a.value].copy())
+
+    def sort_morton(self, bits_per_axis: int = 10, bounds=None, want_map: bool = True):
+        sp = SortParams()
+        sp.bits_per_axis = bits_per_axis
+        if bounds is not None:
+            sp.explicit_bounds = 1
+            for i, v in enumerate(bounds):
+                sp.bounds[i] = float(v)
+        n = self.particle_count()
+        m = np.zeros(max(n, 1), np.uint32) if want_map else None
+        self._chk(self._lib.fp_sort_morton(self._h, C.byref(sp),
+                                           None if m is None else m.ctypes.data_as(_u32)))
+        return None if m is None else m[:n].copy()
+
+    # -- RNG KAT path
+    def philox_blocks(self, counter, key, n: int) -> np.ndarray:
+        c = np.ascontiguousarray(np.asarray(counter, np.uint32))
+        k = np.ascontiguousarray(np.asarray(key, np.uint32))
+        out = np.zeros((int(n), 4), np.uint32)
+        self._chk(self._lib.fp_philox_blocks(self._h, c.ctypes.data_as(_u32),
+                                             k.ctypes.data_as(_u32), int(n),
+                                             out.ctypes.data_as(_u32)))
+        return out
+
+    # -- timing / tuning
+    def profile(self, enable: bool = True):
+        self._chk(self._lib.fp_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self, kernel: int = KERNEL_STEP):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._chk(self._lib.fp_profile_read(self._h, kernel, C.byref(ms), C.byref(n)))
+        return float(ms.value), int(n.value)
+
+    def profile_reset(self):
+        self._chk(self._lib.fp_profile_reset(self._h))
+
+    def set_launch(self, block_size: int = 256, particles_per_thread: int = 1):
+        self._chk(self._lib.fp_set_launch(self._h, block_size, particles_per_thread))
+
+
+__all__ = [
+    "GpuTimestep", "StepConfig", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
+    "library", "ABI", "LIB_PATH",
+]

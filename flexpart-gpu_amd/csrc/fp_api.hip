@@ -1,0 +1,684 @@
+// fp_api.hip - the C ABI (include/flexpart_hip.h) over the gfx950 kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fp_internal.hpp"
+
+static_assert(sizeof(fp_particle) == 96, "fp_particle must match the reference Particle");
+static_assert(sizeof(fp_step_params) == 132, "fp_step_params layout is part of the ABI");
+static_assert(sizeof(fp_ctx_desc) == 64, "fp_ctx_desc layout is part of the ABI");
+static_assert(sizeof(fp_grid_params) == 80, "fp_grid_params layout is part of the ABI");
+static_assert(sizeof(fp_sort_params) == 56, "fp_sort_params layout is part of the ABI");
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(fp_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define FP_HIP(ctx, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? FP_E_NOMEM : FP_E_DEVICE,          \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+#define FP_CHECK(ctx, cond, msg)                                                              \
+    do {                                                                                      \
+        if (!(cond)) return fail((ctx), FP_E_INVALID, (msg));                                 \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, uint64_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+int ensure_current(fp_ctx *ctx, uint64_t a, uint64_t b)
+{
+    b = std::min<uint64_t>(b, ctx->desc.capacity);
+    if (a >= b) return FP_OK;
+    std::vector<fp::Segments::Seg> todo;
+    for (const auto &s : ctx->segs.v)
+        if (s.owner != ctx->cur && s.b > a && s.a < b)
+            todo.push_back(fp::Segments::Seg{std::max(s.a, a), std::min(s.b, b), s.owner});
+    for (const auto &s : todo) {
+        const fp::Soa &src = ctx->soa[s.owner];
+        const fp::Soa &dst = ctx->soa[ctx->cur];
+        for (uint32_t f = 0; f < fp::F_COUNT; ++f)
+            FP_HIP(ctx, hipMemcpyAsync(dst.u(f) + s.a, src.u(f) + s.a, (s.b - s.a) * 4,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->segs.assign(s.a, s.b, ctx->cur);
+    }
+    return FP_OK;
+}
+
+int ensure_stage(fp_ctx *ctx, uint64_t n)
+{
+    if (n <= ctx->aos_stage_cap) return FP_OK;
+    if (ctx->aos_stage) (void)hipFree(ctx->aos_stage);
+    ctx->aos_stage = nullptr;
+    ctx->aos_stage_cap = 0;
+    FP_HIP(ctx, dalloc(&ctx->aos_stage, n));
+    ctx->aos_stage_cap = n;
+    return FP_OK;
+}
+
+int ensure_slot_stage(fp_ctx *ctx, uint64_t n)
+{
+    if (n <= ctx->slot_stage_cap) return FP_OK;
+    if (ctx->slot_stage) (void)hipFree(ctx->slot_stage);
+    ctx->slot_stage = nullptr;
+    ctx->slot_stage_cap = 0;
+    FP_HIP(ctx, dalloc(&ctx->slot_stage, n));
+    ctx->slot_stage_cap = n;
+    return FP_OK;
+}
+
+void prof_begin(fp_ctx *ctx, fp_ctx::Ev &ev, int kernel)
+{
+    ev.kernel = kernel;
+    if (!ctx->profiling) return;
+    if (!ctx->free_events.empty()) {
+        ev = ctx->free_events.back();
+        ctx->free_events.pop_back();
+        ev.kernel = kernel;
+    } else {
+        (void)hipEventCreate(&ev.a);
+        (void)hipEventCreate(&ev.b);
+    }
+    (void)hipEventRecord(ev.a, ctx->stream);
+}
+
+void prof_end(fp_ctx *ctx, fp_ctx::Ev &ev)
+{
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(ev.b, ctx->stream);
+    ctx->events.push_back(ev);
+}
+
+int validate_levels(fp_ctx *ctx, const float *lh)
+{
+    const uint32_t nz = ctx->desc.met_nz;
+    for (uint32_t k = 0; k < nz; ++k)
+        FP_CHECK(ctx, std::isfinite(lh[k]), "level heights must be finite");
+    const bool metres = lh[nz - 1] > 0.0f;
+    if (metres)
+        for (uint32_t k = 1; k < nz; ++k)
+            FP_CHECK(ctx, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
+    ctx->has_heights = metres;
+    ctx->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
+    ctx->level_heights_host.assign(lh, lh + nz);
+    return FP_OK;
+}
+
+void destroy(fp_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    fr(ctx->soa[0].base); fr(ctx->soa[1].base);
+    fr(ctx->met); fr(ctx->level_heights); fr(ctx->met_stage);
+    fr(ctx->pbl); fr(ctx->precip); fr(ctx->pbl_stage);
+    fr(ctx->dry_vd); fr(ctx->wet_lambda); fr(ctx->wet_fraction);
+    fr(ctx->grid_counts); fr(ctx->grid_mass);
+    fr(ctx->aos_stage); fr(ctx->slot_stage);
+    fr(ctx->block_counts); fr(ctx->survivor_map); fr(ctx->active_count_dev);
+    fr(ctx->bounds_partial); fr(ctx->bounds_dev);
+    fr(ctx->sort_keys); fr(ctx->sort_keys_alt); fr(ctx->sort_vals); fr(ctx->sort_vals_alt);
+    fr(ctx->sort_temp);
+    if (ctx->active_count_host) (void)hipHostFree(ctx->active_count_host);
+    for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
+    for (auto &ev : ctx->free_events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fp_version(void) { return "flexpart-hip 0.1 gfx950"; }
+
+const char *fp_last_error(const fp_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
+{
+    if (!out || !desc) return fail(nullptr, FP_E_INVALID, "null argument");
+    *out = nullptr;
+    const fp_ctx_desc &d = *desc;
+    if (d.capacity == 0 || d.capacity > 0x7fffffffull)
+        return fail(nullptr, FP_E_INVALID, "capacity must be in [1, 2^31)");
+    if (d.met_nx == 0 || d.met_ny == 0 || d.met_nz == 0)
+        return fail(nullptr, FP_E_INVALID, "met grid dimensions must be non-zero");
+    if (d.met_nz > fp::kMaxLevels) return fail(nullptr, FP_E_INVALID, "met_nz exceeds 256");
+    const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
+    if (npts * 2 >= 0x80000000ull) return fail(nullptr, FP_E_INVALID, "met grid too large");
+    if (d.n_species < 1 || d.n_species > FP_MAX_SPECIES)
+        return fail(nullptr, FP_E_INVALID, "n_species must be 1..4");
+    const uint64_t ncell = (uint64_t)d.out_nx * d.out_ny * d.out_nz;
+    if (ncell >= 0x80000000ull) return fail(nullptr, FP_E_INVALID, "output grid too large");
+    if (ncell && d.out_nz > FP_MAX_OUTPUT_LEVELS)
+        return fail(nullptr, FP_E_INVALID, "out_nz exceeds 16");
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, FP_E_DEVICE, "no HIP device");
+    if (device_ordinal < 0 || device_ordinal >= ndev)
+        return fail(nullptr, FP_E_INVALID, "device ordinal out of range");
+    e = hipSetDevice(device_ordinal);
+    if (e != hipSuccess) return fail(nullptr, FP_E_DEVICE, hipGetErrorString(e));
+
+    fp_ctx *ctx = new (std::nothrow) fp_ctx();
+    if (!ctx) return fail(nullptr, FP_E_NOMEM, "host allocation failed");
+    ctx->device = device_ordinal;
+    ctx->desc = d;
+    ctx->dispatch = d.capacity;
+    auto bail = [&](int code) {
+        g_create_error = ctx->err;
+        destroy(ctx);
+        return code;
+    };
+#define CREATE_HIP(expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return bail(e_ == hipErrorOutOfMemory ? FP_E_NOMEM : FP_E_DEVICE);                \
+        }                                                                                     \
+    } while (0)
+    CREATE_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        ctx->soa[k].capacity = d.capacity;
+        CREATE_HIP(dalloc(&ctx->soa[k].base, d.capacity * fp::F_COUNT));
+    }
+    CREATE_HIP(hipMemsetAsync(ctx->soa[0].base, 0, d.capacity * fp::F_COUNT * 4, ctx->stream));
+    CREATE_HIP(dalloc(&ctx->met, npts * 2));
+    CREATE_HIP(dalloc(&ctx->level_heights, (uint64_t)d.met_nz));
+    const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
+    CREATE_HIP(dalloc(&ctx->pbl, np));
+    CREATE_HIP(dalloc(&ctx->precip, np));
+    if (ncell) {
+        CREATE_HIP(dalloc(&ctx->grid_counts, ncell));
+        CREATE_HIP(dalloc(&ctx->grid_mass, ncell));
+        CREATE_HIP(hipMemsetAsync(ctx->grid_counts, 0, ncell * 4, ctx->stream));
+        CREATE_HIP(hipMemsetAsync(ctx->grid_mass, 0, ncell * 4, ctx->stream));
+    }
+    CREATE_HIP(dalloc(&ctx->block_counts, fp::compact_blocks((uint32_t)d.capacity) + 1));
+    CREATE_HIP(dalloc(&ctx->survivor_map, d.capacity));
+    CREATE_HIP(dalloc(&ctx->active_count_dev, 1));
+    CREATE_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->active_count_host), sizeof(uint32_t)));
+    CREATE_HIP(dalloc(&ctx->bounds_partial, 2048 * 7));
+    CREATE_HIP(dalloc(&ctx->bounds_dev, 8));
+    CREATE_HIP(dalloc(reinterpret_cast<uint64_t **>(&ctx->sort_keys), d.capacity));
+    CREATE_HIP(dalloc(reinterpret_cast<uint64_t **>(&ctx->sort_keys_alt), d.capacity));
+    CREATE_HIP(dalloc(&ctx->sort_vals, d.capacity));
+    CREATE_HIP(dalloc(&ctx->sort_vals_alt, d.capacity));
+    CREATE_HIP(hipStreamSynchronize(ctx->stream));
+#undef CREATE_HIP
+    ctx->segs.reset(d.capacity, 0);
+    ctx->grid_params.refine_x = 1;
+    ctx->grid_params.refine_y = 1;
+    ctx->grid_params.mass_scale = 1.0e6f;
+    *out = ctx;
+    return FP_OK;
+}
+
+int fp_ctx_destroy(fp_ctx *ctx)
+{
+    destroy(ctx);
+    return FP_OK;
+}
+
+int fp_sync(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- particles
+
+static int check_species(fp_ctx *ctx, const fp_particle *p, uint64_t n)
+{
+    for (uint64_t r = 0; r < n; ++r)
+        for (uint32_t s = ctx->desc.n_species; s < FP_MAX_SPECIES; ++s)
+            if (p[r].mass[s] != 0.0f)
+                return fail(ctx, FP_E_INVALID,
+                            "mass slot beyond n_species is non-zero (slot record " +
+                                std::to_string(r) + ")");
+    return FP_OK;
+}
+
+int fp_particles_upload(fp_ctx *ctx, const fp_particle *p, uint64_t n)
+{
+    FP_CHECK(ctx, ctx && (p || n == 0), "null argument");
+    FP_CHECK(ctx, n <= ctx->desc.capacity, "upload exceeds capacity");
+    if (int rc = check_species(ctx, p, n)) return rc;
+    if (int rc = ensure_stage(ctx, n)) return rc;
+    FP_HIP(ctx, hipMemcpyAsync(ctx->aos_stage, p, n * sizeof(fp_particle), hipMemcpyHostToDevice,
+                               ctx->stream));
+    FP_HIP(ctx, fp::launch_aos_to_soa(ctx->aos_stage, nullptr, n, ctx->s(), ctx->stream));
+    ctx->segs.assign(0, n, ctx->cur);
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // p may be released by the caller
+    return FP_OK;
+}
+
+int fp_particles_upload_slots(fp_ctx *ctx, const uint32_t *slots, const fp_particle *p, uint64_t n)
+{
+    FP_CHECK(ctx, ctx && ((slots && p) || n == 0), "null argument");
+    if (n == 0) return FP_OK;
+    uint64_t lo = UINT64_MAX, hi = 0;
+    for (uint64_t r = 0; r < n; ++r) {
+        FP_CHECK(ctx, slots[r] < ctx->desc.capacity, "slot index out of range");
+        lo = std::min<uint64_t>(lo, slots[r]);
+        hi = std::max<uint64_t>(hi, slots[r]);
+    }
+    if (int rc = check_species(ctx, p, n)) return rc;
+    if (int rc = ensure_current(ctx, lo, hi + 1)) return rc;
+    if (int rc = ensure_stage(ctx, n)) return rc;
+    if (int rc = ensure_slot_stage(ctx, n)) return rc;
+    FP_HIP(ctx, hipMemcpyAsync(ctx->aos_stage, p, n * sizeof(fp_particle), hipMemcpyHostToDevice,
+                               ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(ctx->slot_stage, slots, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, fp::launch_aos_to_soa(ctx->aos_stage, ctx->slot_stage, n, ctx->s(), ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_particles_download(fp_ctx *ctx, fp_particle *out, uint64_t n)
+{
+    FP_CHECK(ctx, ctx && (out || n == 0), "null argument");
+    FP_CHECK(ctx, n <= ctx->desc.capacity, "download exceeds capacity");
+    if (n == 0) return FP_OK;
+    if (int rc = ensure_current(ctx, 0, n)) return rc;
+    if (int rc = ensure_stage(ctx, n)) return rc;
+    FP_HIP(ctx, fp::launch_soa_to_aos(ctx->s(), ctx->aos_stage, n, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(out, ctx->aos_stage, n * sizeof(fp_particle), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_particles_set_dispatch_count(fp_ctx *ctx, uint64_t n)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    FP_CHECK(ctx, n <= ctx->desc.capacity, "dispatch count exceeds capacity");
+    if (int rc = ensure_current(ctx, 0, n)) return rc;
+    ctx->dispatch = n;
+    return FP_OK;
+}
+
+uint64_t fp_particles_dispatch_count(const fp_ctx *ctx) { return ctx ? ctx->dispatch : 0; }
+
+int fp_particles_soa_pointers(fp_ctx *ctx, void **ptrs, uint32_t n_ptrs)
+{
+    FP_CHECK(ctx, ctx && ptrs, "null argument");
+    if (int rc = ensure_current(ctx, 0, ctx->desc.capacity)) return rc;
+    for (uint32_t f = 0; f < n_ptrs && f < fp::F_COUNT; ++f) ptrs[f] = ctx->s().u(f);
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- met / pbl
+
+static int upload_met_common(fp_ctx *ctx, const float *const f[6], hipMemcpyKind kind,
+                             const float *level_heights)
+{
+    const uint64_t npts = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny * ctx->desc.met_nz;
+    if (int rc = validate_levels(ctx, level_heights)) return rc;
+    const float *src[6];
+    if (kind == hipMemcpyHostToDevice) {
+        if (!ctx->met_stage) FP_HIP(ctx, dalloc(&ctx->met_stage, npts * 6));
+        for (int k = 0; k < 6; ++k) {
+            FP_HIP(ctx, hipMemcpyAsync(ctx->met_stage + k * npts, f[k], npts * 4, kind, ctx->stream));
+            src[k] = ctx->met_stage + k * npts;
+        }
+    } else {
+        for (int k = 0; k < 6; ++k) src[k] = f[k];
+    }
+    FP_HIP(ctx, hipMemcpyAsync(ctx->level_heights, ctx->level_heights_host.data(),
+                               ctx->desc.met_nz * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, fp::launch_pack_met(src[0], src[1], src[2], src[3], src[4], src[5], ctx->met,
+                                    (uint32_t)npts, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->met_ready = true;
+    return FP_OK;
+}
+
+int fp_met_upload_bracket(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                          const float *u1, const float *v1, const float *w1,
+                          const float *level_heights)
+{
+    FP_CHECK(ctx, ctx && u0 && v0 && w0 && u1 && v1 && w1 && level_heights, "null argument");
+    const float *f[6] = {u0, v0, w0, u1, v1, w1};
+    return upload_met_common(ctx, f, hipMemcpyHostToDevice, level_heights);
+}
+
+int fp_met_upload_bracket_dev(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                              const float *u1, const float *v1, const float *w1,
+                              const float *level_heights_host)
+{
+    FP_CHECK(ctx, ctx && u0 && v0 && w0 && u1 && v1 && w1 && level_heights_host, "null argument");
+    const float *f[6] = {u0, v0, w0, u1, v1, w1};
+    return upload_met_common(ctx, f, hipMemcpyDeviceToDevice, level_heights_host);
+}
+
+int fp_pbl_upload(fp_ctx *ctx, const float *ustar, const float *wstar, const float *hmix,
+                  const float *oli, const float *precip)
+{
+    FP_CHECK(ctx, ctx && ustar && wstar && hmix && oli, "null argument");
+    const uint64_t np = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny;
+    if (!ctx->pbl_stage) FP_HIP(ctx, dalloc(&ctx->pbl_stage, np * 4));
+    const float *f[4] = {ustar, wstar, hmix, oli};
+    for (int k = 0; k < 4; ++k)
+        FP_HIP(ctx, hipMemcpyAsync(ctx->pbl_stage + k * np, f[k], np * 4, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    FP_HIP(ctx, fp::launch_pack_pbl(ctx->pbl_stage, ctx->pbl, (uint32_t)np, ctx->stream));
+    if (precip) {
+        FP_HIP(ctx, hipMemcpyAsync(ctx->precip, precip, np * 4, hipMemcpyHostToDevice, ctx->stream));
+        ctx->precip_ready = true;
+    }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pbl_ready = true;
+    return FP_OK;
+}
+
+int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,
+                      const float *wet_fraction)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    const uint64_t cap = ctx->desc.capacity;
+    struct { const float *src; float **dst; } items[3] = {
+        {dry_vd, &ctx->dry_vd}, {wet_lambda, &ctx->wet_lambda}, {wet_fraction, &ctx->wet_fraction}};
+    for (auto &it : items) {
+        if (!it.src) continue;
+        if (!*it.dst) FP_HIP(ctx, dalloc(it.dst, cap));
+        FP_HIP(ctx, hipMemcpyAsync(*it.dst, it.src, cap * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- grid
+
+static fp::GridArgs grid_args(const fp_ctx *ctx)
+{
+    fp::GridArgs g{};
+    g.counts = ctx->grid_counts;
+    g.mass_fp = ctx->grid_mass;
+    g.nx = ctx->desc.out_nx;
+    g.ny = ctx->desc.out_ny;
+    g.nz = ctx->desc.out_nz;
+    g.refine_x = ctx->grid_params.refine_x;
+    g.refine_y = ctx->grid_params.refine_y;
+    g.species = ctx->grid_params.species;
+    g.mass_scale = ctx->grid_params.mass_scale;
+    g.metres = g.nz ? (ctx->grid_params.outheights[g.nz - 1] > 0.0f) : 0u;
+    for (int k = 0; k < FP_MAX_OUTPUT_LEVELS; ++k) g.outheights[k] = ctx->grid_params.outheights[k];
+    return g;
+}
+
+int fp_grid_configure(fp_ctx *ctx, const fp_grid_params *params)
+{
+    FP_CHECK(ctx, ctx && params, "null argument");
+    FP_CHECK(ctx, ctx->grid_counts, "context has no output grid");
+    FP_CHECK(ctx, params->refine_x >= 1 && params->refine_y >= 1, "refine factors must be >= 1");
+    FP_CHECK(ctx, params->species < ctx->desc.n_species, "species index beyond n_species");
+    FP_CHECK(ctx, std::isfinite(params->mass_scale), "mass_scale must be finite");
+    ctx->grid_params = *params;
+    ctx->grid_configured = true;
+    return FP_OK;
+}
+
+int fp_grid_clear(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    FP_HIP(ctx, hipMemsetAsync(ctx->grid_counts, 0, ncell * 4, ctx->stream));
+    FP_HIP(ctx, hipMemsetAsync(ctx->grid_mass, 0, ncell * 4, ctx->stream));
+    return FP_OK;
+}
+
+int fp_grid_accumulate(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_GRID);
+    FP_HIP(ctx, fp::launch_grid(ctx->s(), (uint32_t)ctx->dispatch, grid_args(ctx), ctx->stream));
+    prof_end(ctx, ev);
+    return FP_OK;
+}
+
+int fp_grid_download(fp_ctx *ctx, uint32_t *counts, uint32_t *mass_fp)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    if (counts)
+        FP_HIP(ctx, hipMemcpyAsync(counts, ctx->grid_counts, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (mass_fp)
+        FP_HIP(ctx, hipMemcpyAsync(mass_fp, ctx->grid_mass, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- step
+
+int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
+{
+    FP_CHECK(ctx, ctx && p, "null argument");
+    if (!ctx->met_ready) return fail(ctx, FP_E_STATE, "fp_step before fp_met_upload_bracket");
+    if (!ctx->pbl_ready) return fail(ctx, FP_E_STATE, "fp_step before fp_pbl_upload");
+    FP_CHECK(ctx, std::isfinite(p->dt_seconds) && p->dt_seconds > 0.0f, "invalid dt_seconds");
+    FP_CHECK(ctx, p->n_substeps <= 4, "n_substeps must be 0..4");
+    FP_CHECK(ctx, p->direction == 1 || p->direction == -1, "direction must be +1 or -1");
+    FP_CHECK(ctx, p->dry_mode <= FP_DRY_PER_SLOT, "unknown dry_mode");
+    FP_CHECK(ctx, p->wet_mode <= FP_WET_PRECIP_FIELD, "unknown wet_mode");
+    FP_CHECK(ctx, p->loss_mode <= FP_LOSS_MIN_MASS, "unknown loss_mode");
+    if (p->dry_mode == FP_DRY_PER_SLOT && !ctx->dry_vd)
+        return fail(ctx, FP_E_STATE, "per-slot dry deposition without fp_forcing_upload");
+    if (p->wet_mode == FP_WET_PER_SLOT && !(ctx->wet_lambda && ctx->wet_fraction))
+        return fail(ctx, FP_E_STATE, "per-slot wet deposition without fp_forcing_upload");
+    if (p->wet_mode == FP_WET_PRECIP_FIELD && !ctx->precip_ready)
+        return fail(ctx, FP_E_STATE, "precip-field wet deposition without a precip upload");
+    if (p->grid_fused && !ctx->grid_configured)
+        return fail(ctx, FP_E_STATE, "fused gridding before fp_grid_configure");
+
+    fp::StepArgs a{};
+    a.s = ctx->s();
+    a.n = (uint32_t)ctx->dispatch;
+    a.n_species = ctx->desc.n_species;
+    a.met = ctx->met;
+    a.nx = ctx->desc.met_nx;
+    a.ny = ctx->desc.met_ny;
+    a.nz = ctx->desc.met_nz;
+    a.level_heights = ctx->level_heights;
+    a.has_heights = ctx->has_heights ? 1u : 0u;
+    a.zmax = ctx->zmax;
+    a.pbl = ctx->pbl;
+    a.precip = ctx->precip;
+    a.dry_vd_slots = ctx->dry_vd;
+    a.wet_lambda_slots = ctx->wet_lambda;
+    a.wet_fraction_slots = ctx->wet_fraction;
+    a.dt = p->dt_seconds;
+    a.dt_adv = p->direction < 0 ? -p->dt_seconds : p->dt_seconds;
+    a.alpha = std::min(std::max(p->alpha, 0.0f), 1.0f);  // gpu/advection.rs:639
+    a.x_scale = p->x_scale;
+    a.y_scale = p->y_scale;
+    a.z_scale = p->z_scale;
+    a.key0 = p->key[0];
+    a.key1 = p->key[1];
+    a.c0 = p->counter[0];
+    a.c1 = p->counter[1];
+    a.c2 = p->counter[2];
+    a.c3 = p->counter[3];
+    a.n_substeps = p->n_substeps;
+    const uint32_t nv = std::max(p->n_substeps, 1u);
+    a.blocks_per_particle = (2u + nv + 3u) / 4u;  // langevin_fused.wgsl:435-436
+    a.min_height = p->min_height_m;
+    a.rho_grad = p->rho_grad_over_rho;
+    a.dry_mode = p->dry_mode;
+    a.dry_vd = p->dry_vd;
+    a.dry_href = p->dry_href;
+    a.wet_mode = p->wet_mode;
+    a.wet_lambda = p->wet_lambda;
+    a.wet_fraction = p->wet_fraction;
+    a.wet_a = p->wet_a;
+    a.wet_b = p->wet_b;
+    a.loss_mode = p->loss_mode;
+    a.loss_min_mass = p->loss_min_mass;
+    a.grid_fused = p->grid_fused ? 1u : 0u;
+    if (a.grid_fused) a.grid = grid_args(ctx);
+    a.ppt = ctx->ppt;
+
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_STEP);
+    FP_HIP(ctx, fp::launch_step(a, ctx->block_size, ctx->stream));
+    prof_end(ctx, ev);
+
+    if (next_counter) {
+        // host counter advance (gpu/langevin_fused.rs:229-233), 128-bit add
+        const uint64_t off = ctx->dispatch * (p->n_substeps <= 2 ? 1ull : 2ull);
+        const uint64_t lo = (uint64_t)p->counter[0] + (uint32_t)off;
+        const uint64_t mid = (uint64_t)p->counter[1] + (uint32_t)(off >> 32) + (lo >> 32);
+        const uint64_t hi = (uint64_t)p->counter[2] + (mid >> 32);
+        next_counter[0] = (uint32_t)lo;
+        next_counter[1] = (uint32_t)mid;
+        next_counter[2] = (uint32_t)hi;
+        next_counter[3] = p->counter[3] + (uint32_t)(hi >> 32);
+    }
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- compaction / sort
+
+int fp_compact(fp_ctx *ctx, uint64_t *active_count, uint32_t *survivor_map)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    const uint32_t n = (uint32_t)ctx->dispatch;
+    const int other = 1 - ctx->cur;
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_COMPACT);
+    FP_HIP(ctx, fp::launch_compact(ctx->soa[ctx->cur], ctx->soa[other], n, ctx->block_counts,
+                                   ctx->survivor_map, ctx->active_count_dev, ctx->stream));
+    prof_end(ctx, ev);
+    FP_HIP(ctx, hipMemcpyAsync(ctx->active_count_host, ctx->active_count_dev, 4,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t a = *ctx->active_count_host;
+    ctx->segs.assign(0, n, other);
+    ctx->cur = other;
+    ctx->dispatch = a;  // ParticleStore::reset_after_compaction (particles/mod.rs:600-605)
+    if (survivor_map && a)
+        FP_HIP(ctx, hipMemcpy(survivor_map, ctx->survivor_map, (uint64_t)a * 4, hipMemcpyDeviceToHost));
+    if (active_count) *active_count = a;
+    return FP_OK;
+}
+
+int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *p, uint32_t *new_to_old)
+{
+    FP_CHECK(ctx, ctx && p, "null argument");
+    FP_CHECK(ctx, p->bits_per_axis >= 1 && p->bits_per_axis <= 21, "bits_per_axis must be 1..21");
+    const uint32_t n = (uint32_t)ctx->dispatch;
+    if (n == 0) return FP_OK;
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_SORT);
+    FP_HIP(ctx, fp::sort_morton(ctx, *p, n));
+    prof_end(ctx, ev);
+    const int other = 1 - ctx->cur;
+    ctx->segs.assign(0, n, other);
+    ctx->cur = other;
+    if (new_to_old) {
+        FP_HIP(ctx, hipMemcpyAsync(new_to_old, ctx->sort_vals_alt, (uint64_t)n * 4,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- philox
+
+int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[2], uint64_t n,
+                     uint32_t *out)
+{
+    FP_CHECK(ctx, ctx && counter && key && (out || n == 0), "null argument");
+    if (n == 0) return FP_OK;
+    uint32_t *d = nullptr;
+    FP_HIP(ctx, dalloc(&d, n * 4));
+    hipError_t e = fp::launch_philox_blocks(counter, key, n, d, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    FP_HIP(ctx, e);
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------- timing / tuning
+
+int fp_profile_enable(fp_ctx *ctx, int enable)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    ctx->profiling = enable != 0;
+    return FP_OK;
+}
+
+static int collect_events(fp_ctx *ctx)
+{
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &ev : ctx->events) {
+        float ms = 0.0f;
+        FP_HIP(ctx, hipEventElapsedTime(&ms, ev.a, ev.b));
+        ctx->prof_ms[ev.kernel] += ms;
+        ctx->prof_n[ev.kernel] += 1;
+        ctx->free_events.push_back(ev);
+    }
+    ctx->events.clear();
+    return FP_OK;
+}
+
+int fp_profile_read(fp_ctx *ctx, int kernel, double *total_ms, uint64_t *launches)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    FP_CHECK(ctx, kernel >= 0 && kernel < FP_KERNEL_COUNT, "unknown kernel id");
+    if (int rc = collect_events(ctx)) return rc;
+    if (total_ms) *total_ms = ctx->prof_ms[kernel];
+    if (launches) *launches = ctx->prof_n[kernel];
+    return FP_OK;
+}
+
+int fp_profile_reset(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    if (int rc = collect_events(ctx)) return rc;
+    for (int k = 0; k < FP_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0.0; ctx->prof_n[k] = 0; }
+    return FP_OK;
+}
+
+int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_thread)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    FP_CHECK(ctx, block_size == 64 || block_size == 128 || block_size == 256 || block_size == 512 ||
+                      block_size == 1024,
+             "block_size must be 64, 128, 256, 512 or 1024");
+    FP_CHECK(ctx, particles_per_thread >= 1 && particles_per_thread <= 16,
+             "particles_per_thread must be 1..16");
+    ctx->block_size = block_size;
+    ctx->ppt = particles_per_thread;
+    return FP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+// fp_aux.hip - data movement kernels around the step: met/PBL packing,
+// AoS <-> SoA transposes, stable stream compaction and the Philox
+// known-answer kernel.
+#include "fp_internal.hpp"
+
+namespace fp {
+namespace {
+
+constexpr uint32_t FLAG_ACTIVE = 1u;
+
+// 32-byte met record per grid point: {u0, v0, w0, u1, v1, w1, 0, 0}.  One
+// record holds both time levels of all three components, so a trilinear
+// corner is two 16-byte loads from one cache line.
+__global__ void pack_met_kernel(const float *u0, const float *v0, const float *w0,
+                                const float *u1, const float *v1, const float *w1, float4 *met,
+                                uint32_t npts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    met[2u * i] = make_float4(u0[i], v0[i], w0[i], u1[i]);
+    met[2u * i + 1u] = make_float4(v1[i], w1[i], 0.0f, 0.0f);
+}
+
+__global__ void pack_pbl_kernel(const float *stage, float4 *pbl, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pbl[i] = make_float4(stage[i], stage[n + i], stage[2u * n + i], stage[3u * n + i]);
+}
+
+// AoS (96 B records) -> SoA.  slots == nullptr writes record r to slot r.
+__global__ void aos_to_soa_kernel(const uint32_t *aos, const uint32_t *slots, uint64_t n, Soa s)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t slot = slots ? slots[r] : r;
+    const uint32_t *src = aos + r * F_COUNT;
+#pragma unroll
+    for (uint32_t f = 0; f < F_COUNT; ++f) s.u(f)[slot] = src[f];
+}
+
+__global__ void soa_to_aos_kernel(Soa s, uint32_t *aos, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t *dst = aos + r * F_COUNT;
+#pragma unroll
+    for (uint32_t f = 0; f < F_COUNT; ++f) dst[f] = s.u(f)[r];
+}
+
+// ---------------------------------------------------------------- compaction
+//
+// Reduce-then-scan over the flag array (no inter-block waiting):
+//   1. count_kernel  : active slots per tile of kTile slots
+//   2. scan_kernel   : exclusive scan of the tile counts (one block), total
+//   3. scatter_kernel: each tile ranks its active slots with wave ballots and
+//                      copies them to offset + rank; slots at or beyond the
+//                      total receive a deactivated copy of themselves
+// The survivor order is ascending slot order, as in compaction.wgsl.
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kRows = 4;
+constexpr uint32_t kTile = kBlock * kRows;
+constexpr uint32_t kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ uint32_t active_at(const Soa &s, uint64_t i, uint32_t n)
+{
+    return (i < n) ? (s.u(F_FLAGS)[i] & FLAG_ACTIVE) : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) count_kernel(Soa s, uint32_t n, uint32_t *tile_counts)
+{
+    __shared__ uint32_t wsum[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) c += active_at(s, base + r * kBlock + threadIdx.x, n);
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+    if (__lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWavesPerBlock; ++w) t += wsum[w];
+        tile_counts[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of ntiles counts in place; writes the total.
+__global__ void __launch_bounds__(1024) scan_kernel(uint32_t *tile_counts, uint32_t ntiles,
+                                                    uint32_t *total)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (ntiles + 1023u) / 1024u;
+    const uint32_t lo = t * per, hi = min(lo + per, ntiles);
+    uint32_t sum = 0;
+    for (uint32_t k = lo; k < hi; ++k) sum += tile_counts[k];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t c = tile_counts[k];
+        tile_counts[k] = run;
+        run += c;
+    }
+    if (t == 1023u) *total = part[1023];
+}
+
+__device__ __forceinline__ void copy_slot(const Soa &in, const Soa &out, uint64_t src, uint64_t dst,
+                                          bool deactivate)
+{
+#pragma unroll
+    for (uint32_t f = 0; f < F_COUNT; ++f) {
+        uint32_t v = in.u(f)[src];
+        if (f == F_FLAGS && deactivate) v &= ~FLAG_ACTIVE;
+        out.u(f)[dst] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) scatter_kernel(Soa in, Soa out, uint32_t n,
+                                                         const uint32_t *tile_offsets,
+                                                         const uint32_t *total_p,
+                                                         uint32_t *survivor_map)
+{
+    __shared__ uint32_t wcount[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const uint32_t total = *total_p;
+    const uint32_t wave = threadIdx.x / kWave, lane = __lane_id();
+    uint32_t running = tile_offsets[blockIdx.x];
+    for (uint32_t r = 0; r < kRows; ++r) {
+        const uint64_t i = base + r * kBlock + threadIdx.x;
+        const uint32_t a = active_at(in, i, n);
+        const uint64_t bal = __ballot(a != 0u);
+        const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t woff = 0, rowtot = 0;
+        for (uint32_t w = 0; w < kWavesPerBlock; ++w) {
+            if (w < wave) woff += wcount[w];
+            rowtot += wcount[w];
+        }
+        __syncthreads();
+        if (a) {
+            const uint32_t dst = running + woff + below;
+            copy_slot(in, out, i, dst, false);
+            survivor_map[dst] = (uint32_t)i;
+        }
+        if (i < n && i >= total) copy_slot(in, out, i, i, true);
+        running += rowtot;
+    }
+}
+
+__global__ void philox_kernel(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0_in,
+                              uint32_t k1_in, uint64_t n, uint32_t *out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t lo = (uint64_t)c0 + (uint32_t)i;
+    const uint64_t mid = (uint64_t)c1 + (uint32_t)(i >> 32) + (lo >> 32);
+    const uint64_t hi = (uint64_t)c2 + (mid >> 32);
+    uint32_t x = (uint32_t)lo, y = (uint32_t)mid, z = (uint32_t)hi, w = c3 + (uint32_t)(hi >> 32);
+    uint32_t k0 = k0_in, k1 = k1_in;
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t h0 = __umulhi(0xD2511F53u, x), l0 = 0xD2511F53u * x;
+        const uint32_t h1 = __umulhi(0xCD9E8D57u, z), l1 = 0xCD9E8D57u * z;
+        x = h1 ^ y ^ k0; y = l1; z = h0 ^ w ^ k1; w = l0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[4 * i + 0] = x; out[4 * i + 1] = y; out[4 * i + 2] = z; out[4 * i + 3] = w;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
+                           const float *v1, const float *w1, float4 *met, uint32_t npts,
+                           hipStream_t st)
+{
+    pack_met_kernel<<<blocks_for(npts, 256), 256, 0, st>>>(u0, v0, w0, u1, v1, w1, met, npts);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_pbl(const float *stage, float4 *pbl, uint32_t n, hipStream_t st)
+{
+    pack_pbl_kernel<<<blocks_for(n, 256), 256, 0, st>>>(stage, pbl, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const fp_particle *aos, const uint32_t *slots, uint64_t n, Soa s,
+                             hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    aos_to_soa_kernel<<<blocks_for(n, 256), 256, 0, st>>>(
+        reinterpret_cast<const uint32_t *>(aos), slots, n, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(Soa s, fp_particle *aos, uint64_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    soa_to_aos_kernel<<<blocks_for(n, 256), 256, 0, st>>>(s, reinterpret_cast<uint32_t *>(aos), n);
+    return hipGetLastError();
+}
+
+uint64_t compact_blocks(uint32_t n) { return (n + kTile - 1) / kTile; }
+
+hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *tile_counts,
+                          uint32_t *survivor_map, uint32_t *active_count, hipStream_t st)
+{
+    const uint32_t ntiles = (uint32_t)compact_blocks(n);
+    if (ntiles == 0) return hipMemsetAsync(active_count, 0, sizeof(uint32_t), st);
+    count_kernel<<<ntiles, kBlock, 0, st>>>(in, n, tile_counts);
+    scan_kernel<<<1, 1024, 0, st>>>(tile_counts, ntiles, active_count);
+    scatter_kernel<<<ntiles, kBlock, 0, st>>>(in, out, n, tile_counts, active_count, survivor_map);
+    return hipGetLastError();
+}
+
+hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2], uint64_t n,
+                                uint32_t *out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    philox_kernel<<<blocks_for(n, 256), 256, 0, st>>>(counter[0], counter[1], counter[2],
+                                                      counter[3], key[0], key[1], n, out);
+    return hipGetLastError();
+}
+
+}  // namespace fp

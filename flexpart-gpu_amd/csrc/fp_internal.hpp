@@ -1,0 +1,211 @@
+// fp_internal.hpp - device data layout, kernel argument blocks and the
+// context object behind the C ABI (include/flexpart_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "flexpart_hip.h"
+
+namespace fp {
+
+// SoA particle state: field f of slot i lives at base[f * capacity + i].  The
+// field order is the reference's 96-byte Particle (particles/mod.rs:76-168),
+// so an AoS <-> SoA transpose moves 32-bit word f of particle i to array f.
+enum Field : uint32_t {
+    F_POS_X = 0, F_POS_Y, F_POS_Z, F_CELL_X, F_CELL_Y, F_FLAGS,
+    F_MASS0, F_MASS1, F_MASS2, F_MASS3,
+    F_VEL_U, F_VEL_V, F_VEL_W, F_TURB_U, F_TURB_V, F_TURB_W,
+    F_TIME, F_TIMESTEP, F_TIME_MEM, F_TIME_SPLIT, F_RELEASE_POINT, F_CLASS_ID, F_CBT, F_PAD0,
+    F_COUNT
+};
+static_assert(F_COUNT * 4 == sizeof(fp_particle), "particle word count");
+
+constexpr uint32_t kMaxLevels = 256;   // met levels held in LDS by the step kernel
+constexpr uint32_t kWave = 64;         // CDNA wavefront width
+constexpr uint32_t kXcds = 8;          // MI355X accelerator complex dies
+
+// Pointers of one SoA set.
+struct Soa {
+    uint32_t *base = nullptr;  // F_COUNT * capacity words
+    uint64_t capacity = 0;
+    __host__ __device__ float *f(uint32_t field) const {
+        return reinterpret_cast<float *>(base + (uint64_t)field * capacity);
+    }
+    __host__ __device__ int32_t *i(uint32_t field) const {
+        return reinterpret_cast<int32_t *>(base + (uint64_t)field * capacity);
+    }
+    __host__ __device__ uint32_t *u(uint32_t field) const { return base + (uint64_t)field * capacity; }
+};
+
+// Concentration-grid configuration shared by the fused and standalone paths.
+struct GridArgs {
+    uint32_t *counts;
+    uint32_t *mass_fp;
+    uint32_t nx, ny, nz;
+    uint32_t refine_x, refine_y;
+    uint32_t species;
+    float mass_scale;
+    uint32_t metres;  // outheights[nz-1] > 0
+    float outheights[FP_MAX_OUTPUT_LEVELS];
+};
+
+// Everything the fused step kernel reads, passed by value.
+struct StepArgs {
+    Soa s;
+    uint32_t n;             // dispatch count
+    uint32_t n_species;
+    // met: packed 32-byte records {u0,v0,w0,u1,v1,w1,0,0} per grid point
+    const float4 *met;
+    uint32_t nx, ny, nz;
+    const float *level_heights;  // nz floats (device)
+    uint32_t has_heights;
+    float zmax;
+    // PBL: packed {ustar, wstar, hmix, oli} per (i, j); precip optional
+    const float4 *pbl;
+    const float *precip;
+    // forcing
+    const float *dry_vd_slots, *wet_lambda_slots, *wet_fraction_slots;
+    // scalars
+    float dt_adv, dt, alpha, x_scale, y_scale, z_scale;
+    uint32_t key0, key1, c0, c1, c2, c3;
+    uint32_t n_substeps, blocks_per_particle;
+    float min_height, rho_grad;
+    uint32_t dry_mode;
+    float dry_vd, dry_href;
+    uint32_t wet_mode;
+    float wet_lambda, wet_fraction, wet_a, wet_b;
+    uint32_t loss_mode;
+    float loss_min_mass;
+    uint32_t grid_fused;
+    GridArgs grid;
+    uint32_t ppt;           // particles per thread
+};
+
+// Which of the two SoA sets holds the current data of each slot range.
+// Compaction and sort write the dispatch range into the other set and swap;
+// slots beyond the dispatch range keep their data in the set that last wrote
+// them and are copied over lazily when a download, an upload or a wider
+// dispatch range needs them.
+struct Segments {
+    struct Seg { uint64_t a, b; int owner; };
+    std::vector<Seg> v;
+    void reset(uint64_t cap, int owner) { v.assign(1, Seg{0, cap, owner}); }
+    void assign(uint64_t a, uint64_t b, int owner)
+    {
+        if (a >= b) return;
+        std::vector<Seg> out;
+        for (const Seg &s : v) {
+            if (s.b <= a || s.a >= b) { out.push_back(s); continue; }
+            if (s.a < a) out.push_back(Seg{s.a, a, s.owner});
+            if (s.b > b) out.push_back(Seg{b, s.b, s.owner});
+        }
+        out.push_back(Seg{a, b, owner});
+        std::sort(out.begin(), out.end(), [](const Seg &x, const Seg &y) { return x.a < y.a; });
+        v.clear();
+        for (const Seg &s : out) {
+            if (!v.empty() && v.back().owner == s.owner && v.back().b == s.a) v.back().b = s.b;
+            else v.push_back(s);
+        }
+    }
+};
+
+}  // namespace fp
+
+struct fp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    fp_ctx_desc desc{};
+    uint64_t dispatch = 0;
+
+    fp::Soa soa[2];
+    int cur = 0;
+    fp::Segments segs;
+
+    // met
+    float4 *met = nullptr;         // 2 float4 per point
+    float *level_heights = nullptr;
+    std::vector<float> level_heights_host;
+    bool met_ready = false;
+    bool has_heights = false;
+    float zmax = 0.0f;
+    float *met_stage = nullptr;    // 6 * npoints floats for host uploads
+
+    // PBL
+    float4 *pbl = nullptr;
+    float *precip = nullptr;
+    bool pbl_ready = false;
+    bool precip_ready = false;
+    float *pbl_stage = nullptr;    // 4 * nx*ny
+
+    // forcing
+    float *dry_vd = nullptr, *wet_lambda = nullptr, *wet_fraction = nullptr;
+
+    // grid
+    uint32_t *grid_counts = nullptr, *grid_mass = nullptr;
+    fp_grid_params grid_params{};
+    bool grid_configured = false;
+
+    // staging for AoS exchange
+    fp_particle *aos_stage = nullptr;
+    uint64_t aos_stage_cap = 0;
+    uint32_t *slot_stage = nullptr;
+    uint64_t slot_stage_cap = 0;
+
+    // compaction
+    uint32_t *block_counts = nullptr;
+    uint64_t block_counts_cap = 0;
+    uint32_t *survivor_map = nullptr;
+    uint32_t *active_count_dev = nullptr;
+    uint32_t *active_count_host = nullptr;  // pinned
+
+    // sort
+    double *bounds_partial = nullptr;
+    uint64_t bounds_partial_cap = 0;
+    double *bounds_dev = nullptr;
+    void *sort_keys = nullptr, *sort_keys_alt = nullptr;
+    uint32_t *sort_vals = nullptr, *sort_vals_alt = nullptr;
+    void *sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+
+    // launch shape
+    uint32_t block_size = 256;
+    uint32_t ppt = 1;
+
+    // profiling
+    bool profiling = false;
+    struct Ev { hipEvent_t a, b; int kernel; };
+    std::vector<Ev> events;
+    std::vector<Ev> free_events;
+    double prof_ms[FP_KERNEL_COUNT] = {0, 0, 0, 0};
+    uint64_t prof_n[FP_KERNEL_COUNT] = {0, 0, 0, 0};
+
+    const fp::Soa &s() const { return soa[cur]; }
+};
+
+namespace fp {
+
+// Kernel launchers (fp_kernels.hip).
+hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
+hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
+                           const float *v1, const float *w1, float4 *met, uint32_t npts,
+                           hipStream_t st);
+hipError_t launch_pack_pbl(const float *stage, float4 *pbl, uint32_t n, hipStream_t st);
+hipError_t launch_aos_to_soa(const fp_particle *aos, const uint32_t *slots, uint64_t n, Soa s,
+                             hipStream_t st);
+hipError_t launch_soa_to_aos(Soa s, fp_particle *aos, uint64_t n, hipStream_t st);
+hipError_t launch_grid(const Soa &s, uint32_t n, const GridArgs &g, hipStream_t st);
+hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *block_counts,
+                          uint32_t *survivor_map, uint32_t *active_count, hipStream_t st);
+uint64_t compact_blocks(uint32_t n);
+hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2], uint64_t n,
+                                uint32_t *out, hipStream_t st);
+// Morton sort: keys + stable radix sort + gather (fp_sort.hip).
+hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n);
+
+}  // namespace fp

@@ -1,0 +1,197 @@
+// fp_sort.hip - Morton (Z-order) re-sort of the particle slots on the device.
+//
+// ParticleStore::sort_spatially (particles/mod.rs:649-699) on the host
+// becomes: an f64 min/max reduction of the active positions (the
+// ActiveParticleExtent bounds), a key per slot (quantise each axis to
+// bits_per_axis bins, interleave x,y,z; inactive slots get a sentinel above
+// every active key), a stable LSD radix sort of (key, slot) pairs over only
+// the 3*bits+1 key bits, and a gather of every SoA field through the sorted
+// slot list.  Stability plus the sentinel reproduce the reference order
+// (active by key, ties and inactive slots in original order).
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "fp_internal.hpp"
+
+namespace fp {
+namespace {
+
+constexpr uint32_t FLAG_ACTIVE = 1u;
+constexpr uint32_t kRB = 256;
+
+__device__ __forceinline__ void slot_pos(const Soa &s, uint64_t i, double &x, double &y, double &z)
+{
+    x = (double)s.i(F_CELL_X)[i] + (double)s.f(F_POS_X)[i];
+    y = (double)s.i(F_CELL_Y)[i] + (double)s.f(F_POS_Y)[i];
+    z = (double)s.f(F_POS_Z)[i];
+}
+
+// Per-block min/max of active positions: out[b*7 + {0..5}] = xmin,xmax,ymin,
+// ymax,zmin,zmax; out[b*7+6] = 1 if the block saw an active slot.
+__global__ void __launch_bounds__(kRB) bounds_partial_kernel(Soa s, uint32_t n, double *out)
+{
+    __shared__ double red[6][kRB];
+    __shared__ uint32_t any;
+    if (threadIdx.x == 0) any = 0;
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    bool seen = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kRB) {
+        if (!(s.u(F_FLAGS)[i] & FLAG_ACTIVE)) continue;
+        double x, y, z;
+        slot_pos(s, i, x, y, z);
+        v[0] = fmin(v[0], x); v[1] = fmax(v[1], x);
+        v[2] = fmin(v[2], y); v[3] = fmax(v[3], y);
+        v[4] = fmin(v[4], z); v[5] = fmax(v[5], z);
+        seen = true;
+    }
+    __syncthreads();
+    if (seen) atomicOr(&any, 1u);
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (uint32_t off = kRB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int k = 0; k < 6; k += 2) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + off]);
+                red[k + 1][threadIdx.x] = fmax(red[k + 1][threadIdx.x], red[k + 1][threadIdx.x + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; ++k) out[blockIdx.x * 7 + k] = red[k][0];
+        out[blockIdx.x * 7 + 6] = any ? 1.0 : 0.0;
+    }
+}
+
+__global__ void bounds_final_kernel(const double *part, uint32_t nparts, double *bounds)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    double any = 0.0;
+    for (uint32_t b = 0; b < nparts; ++b) {
+        const double *p = part + b * 7;
+        if (p[6] == 0.0) continue;
+        any = 1.0;
+        for (int k = 0; k < 6; k += 2) {
+            v[k] = fmin(v[k], p[k]);
+            v[k + 1] = fmax(v[k + 1], p[k + 1]);
+        }
+    }
+    for (int k = 0; k < 6; ++k) bounds[k] = v[k];
+    bounds[6] = any;
+}
+
+// quantize_axis (particles/mod.rs:415-425)
+__device__ __forceinline__ uint32_t quantize(double v, double lo, double hi, uint32_t max_bin)
+{
+    if (!isfinite(v) || !isfinite(lo) || !isfinite(hi)) return 0u;
+    const double span = hi - lo;
+    if (span <= 2.220446049250313e-16) return 0u;
+    double t = (v - lo) / span;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return (uint32_t)floor(t * (double)max_bin);
+}
+
+// Spread the low 21 bits of v so that bit b moves to bit 3b.
+__device__ __forceinline__ uint64_t spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+template <typename K>
+__global__ void keys_kernel(Soa s, uint32_t n, const double *bounds, uint32_t bits, K *keys,
+                            uint32_t *vals)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kRB + threadIdx.x;
+    if (i >= n) return;
+    const K sentinel = (K)1 << (3u * bits);
+    K key = sentinel;
+    if ((s.u(F_FLAGS)[i] & FLAG_ACTIVE) && bounds[6] != 0.0) {
+        double x, y, z;
+        slot_pos(s, i, x, y, z);
+        const uint32_t mb = (1u << bits) - 1u;
+        const uint64_t m = spread3(quantize(x, bounds[0], bounds[1], mb)) |
+                           (spread3(quantize(y, bounds[2], bounds[3], mb)) << 1) |
+                           (spread3(quantize(z, bounds[4], bounds[5], mb)) << 2);
+        key = (K)m;
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void gather_kernel(Soa in, Soa out, uint32_t n, const uint32_t *new_to_old)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * kRB + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t src = new_to_old[j];
+#pragma unroll
+    for (uint32_t f = 0; f < F_COUNT; ++f) out.u(f)[j] = in.u(f)[src];
+}
+
+template <typename K>
+hipError_t sort_pairs(fp_ctx *ctx, uint32_t n, uint32_t end_bit)
+{
+    K *k_in = static_cast<K *>(ctx->sort_keys);
+    K *k_out = static_cast<K *>(ctx->sort_keys_alt);
+    size_t need = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, need, k_in, k_out, ctx->sort_vals,
+                                             ctx->sort_vals_alt, (size_t)n, 0u, end_bit, ctx->stream);
+    if (e != hipSuccess) return e;
+    if (need > ctx->sort_temp_bytes) {
+        if (ctx->sort_temp) (void)hipFree(ctx->sort_temp);
+        ctx->sort_temp = nullptr;
+        e = hipMalloc(&ctx->sort_temp, need);
+        if (e != hipSuccess) return e;
+        ctx->sort_temp_bytes = need;
+    }
+    return rocprim::radix_sort_pairs(ctx->sort_temp, need, k_in, k_out, ctx->sort_vals,
+                                     ctx->sort_vals_alt, (size_t)n, 0u, end_bit, ctx->stream);
+}
+
+}  // namespace
+
+// Sorts slots [0, n) of the current set into the other set; the caller swaps
+// sets.  On return ctx->sort_vals_alt holds new_to_old.
+hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n)
+{
+    hipStream_t st = ctx->stream;
+    const Soa in = ctx->soa[ctx->cur];
+    const Soa out = ctx->soa[1 - ctx->cur];
+    hipError_t e;
+    if (p.explicit_bounds) {
+        double b[7];
+        for (int k = 0; k < 6; ++k) b[k] = p.bounds[k];
+        b[6] = 1.0;
+        e = hipMemcpyAsync(ctx->bounds_dev, b, sizeof b, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);  // b lives on this stack frame
+        if (e != hipSuccess) return e;
+    } else {
+        const uint32_t nparts = (uint32_t)std::min<uint64_t>((n + kRB - 1) / kRB, 2048);
+        bounds_partial_kernel<<<nparts, kRB, 0, st>>>(in, n, ctx->bounds_partial);
+        bounds_final_kernel<<<1, 64, 0, st>>>(ctx->bounds_partial, nparts, ctx->bounds_dev);
+    }
+    const uint32_t grid = (n + kRB - 1) / kRB;
+    const uint32_t end_bit = 3u * p.bits_per_axis + 1u;
+    if (end_bit <= 32u) {
+        keys_kernel<uint32_t><<<grid, kRB, 0, st>>>(in, n, ctx->bounds_dev, p.bits_per_axis,
+                                                    static_cast<uint32_t *>(ctx->sort_keys),
+                                                    ctx->sort_vals);
+        e = sort_pairs<uint32_t>(ctx, n, end_bit);
+    } else {
+        keys_kernel<uint64_t><<<grid, kRB, 0, st>>>(in, n, ctx->bounds_dev, p.bits_per_axis,
+                                                    static_cast<uint64_t *>(ctx->sort_keys),
+                                                    ctx->sort_vals);
+        e = sort_pairs<uint64_t>(ctx, n, end_bit);
+    }
+    if (e != hipSuccess) return e;
+    gather_kernel<<<grid, kRB, 0, st>>>(in, out, n, ctx->sort_vals_alt);
+    return hipGetLastError();
+}
+
+}  // namespace fp

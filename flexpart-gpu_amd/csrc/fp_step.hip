@@ -1,0 +1,555 @@
+// fp_step.hip - the fused per-particle timestep for gfx950.
+//
+// One kernel replaces the reference's four production dispatches
+// (advection_dual_wind.wgsl, langevin_fused.wgsl, dry_deposition.wgsl,
+// wet_deposition.wgsl; step order of particle_step.wgsl:684-883).  Each
+// particle's hot state (44 B) is read once from SoA arrays with coalesced
+// loads, advanced through advection, Hanna turbulence, Philox/Box-Muller
+// noise, the Langevin update with vertical sub-steps and PBL reflection, dry
+// and wet deposition, and written back once (56 B with the velocity memory).
+// Optionally the concentration grid is accumulated in the same pass.
+//
+// Floating point: compiled with -ffp-contract=off and correctly rounded
+// divide/sqrt; transcendentals come from fp_math.h.  With the operation order
+// kept as in the reference shaders, results are reproducible bit for bit.
+#include "fp_internal.hpp"
+#include "fp_math.h"
+
+namespace fp {
+namespace {
+
+constexpr uint32_t FLAG_ACTIVE = 1u;
+constexpr float USTAR_MIN = 1.0e-4f;
+constexpr float TLU_MIN = 10.0f;
+constexpr float TLV_MIN = 10.0f;
+constexpr float TLW_MIN = 30.0f;
+constexpr float SIGMA_FLOOR = 1.0e-2f;
+constexpr float H_MIN = 1.0f;
+constexpr float F32_MAX = 3.40282347e38f;
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ bool finite_f(float v) { return (v == v) && (fabsf(v) <= F32_MAX); }
+__device__ __forceinline__ float pos_or_zero(float v) { return (finite_f(v) && v > 0.0f) ? v : 0.0f; }
+__device__ __forceinline__ float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// ---------------------------------------------------------------- Philox
+
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// base + offset, 128-bit wrapping (physics/rng.rs:69-87)
+__device__ __forceinline__ U4 ctr_add(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t off)
+{
+    const uint64_t lo = (uint64_t)c0 + (uint32_t)off;
+    const uint64_t mid = (uint64_t)c1 + (uint32_t)(off >> 32) + (lo >> 32);
+    const uint64_t hi = (uint64_t)c2 + (mid >> 32);
+    return U4{(uint32_t)lo, (uint32_t)mid, (uint32_t)hi, c3 + (uint32_t)(hi >> 32)};
+}
+
+__device__ __forceinline__ float unit24(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
+
+__device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, float &g1)
+{
+    const float ru = clampf(unit24(a), 1.1754944e-38f, 0.99999994f);
+    const float au = clampf(unit24(b), 0.0f, 0.99999994f);
+    const float rad = fpm_sqrtf(-2.0f * fpm_logf(ru));
+    float sn, cs;
+    fpm_sincosf(6.2831855f * au, &sn, &cs);
+    g0 = rad * cs;
+    g1 = rad * sn;
+}
+
+// ---------------------------------------------------------------- met sampling
+
+struct Axis { uint32_t lo, hi; float t; };
+
+__device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
+{
+    const float c = clampf(coord, 0.0f, (float)(n - 1));
+    const float fl = floorf(c);
+    const uint32_t l = (uint32_t)fl;
+    return Axis{l, min(l + 1u, n - 1u), c - fl};
+}
+
+// z [m] -> fractional level.  First k >= 1 with z <= h[k] by bisection over
+// the non-decreasing level table in LDS (same result as the linear scan of
+// advection_dual_wind.wgsl:94-117).
+__device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz, bool heights)
+{
+    if (!heights) return z;
+    const float h0 = lh[0];
+    if (z <= h0) return h0 > 0.0f ? z / h0 : 0.0f;
+    uint32_t lo = 1, hi = nz;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (z <= lh[mid]) hi = mid; else lo = mid + 1;
+    }
+    if (lo >= nz) return (float)(nz - 1);
+    const float hp = lh[lo - 1];
+    const float den = lh[lo] - hp;
+    return den > 0.0f ? (float)(lo - 1) + (z - hp) / den : (float)(lo - 1);
+}
+
+struct Wind { float u, v, w; };
+
+__device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, float x, float y, float z)
+{
+    const Axis ax = axis_of(x, a.nx);
+    const Axis ay = axis_of(y, a.ny);
+    const Axis az = axis_of(level_of(z, lh, a.nz, a.has_heights != 0), a.nz);
+    const uint32_t ny = a.ny, nz = a.nz;
+    const uint32_t r00 = (ax.lo * ny + ay.lo) * nz, r10 = (ax.hi * ny + ay.lo) * nz;
+    const uint32_t r01 = (ax.lo * ny + ay.hi) * nz, r11 = (ax.hi * ny + ay.hi) * nz;
+    const uint32_t idx[8] = {r00 + az.lo, r10 + az.lo, r01 + az.lo, r11 + az.lo,
+                             r00 + az.hi, r10 + az.hi, r01 + az.hi, r11 + az.hi};
+    float c[8][6];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float4 p = a.met[2u * idx[k]];
+        const float4 q = a.met[2u * idx[k] + 1u];
+        c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
+        c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
+    }
+    float v[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const float x0 = mix(c[0][f], c[1][f], ax.t), x1 = mix(c[2][f], c[3][f], ax.t);
+        const float x2 = mix(c[4][f], c[5][f], ax.t), x3 = mix(c[6][f], c[7][f], ax.t);
+        v[f] = mix(mix(x0, x1, ay.t), mix(x2, x3, ay.t), az.t);
+    }
+    return Wind{mix(v[0], v[3], a.alpha), mix(v[1], v[4], a.alpha), mix(v[2], v[5], a.alpha)};
+}
+
+__device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
+{
+    const float cl = clampf(c, 0.0f, (float)(n - 1));
+    uint32_t ci = (uint32_t)floorf(cl);
+    float fr = cl - (float)ci;
+    if (ci >= n - 1u) { ci = n - 1u; fr = 0.0f; }
+    cell = (int32_t)ci;
+    frac = fr;
+}
+
+// ---------------------------------------------------------------- Hanna
+
+struct Turb {
+    float ust, wst, ol, h;
+    float sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
+};
+
+__device__ __forceinline__ bool neutral_regime(float ol, float h)
+{
+    return !finite_f(ol) || ol == 0.0f || h / fabsf(ol) < 1.0f;
+}
+
+// Vertical profile (sigma_w, d sigma_w/dz, T_Lw) at height z: the w part of
+// compute_hanna (langevin_fused.wgsl:224-293) and all of hanna_short_w
+// (:296-344), which share their formulas.
+__device__ __forceinline__ void vertical_profile(float ust, float wst, float ol, float h, float z,
+                                                 float zeta, float &sigw, float &dsigwdz, float &tlw)
+{
+    if (neutral_regime(ol, h)) {
+        const float corr = z / ust;
+        sigw = 1.3f * ust * fpm_expf(-2.0e-4f * corr);
+        dsigwdz = -2.0e-4f * sigw;
+        sigw = sigw + SIGMA_FLOOR;
+        tlw = 0.5f * z / sigw / (1.0f + 1.5e-3f * corr);
+    } else if (ol < 0.0f) {
+        const float p23 = fpm_powf(zeta, 2.0f / 3.0f);
+        const float sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * p23 + (1.8f - 1.4f * zeta) * ust * ust;
+        sigw = fpm_sqrtf(sq) + SIGMA_FLOOR;
+        dsigwdz = 0.5f / sigw / h *
+                  (-1.4f * ust * ust +
+                   wst * wst * (0.8f * fpm_powf(fmaxf(zeta, 1.0e-3f), -1.0f / 3.0f) - 1.8f * p23));
+        if (z < fabsf(ol)) tlw = 0.1f * z / (sigw * (0.55f - 0.38f * fabsf(z / ol)));
+        else if (zeta < 0.1f) tlw = 0.59f * z / sigw;
+        else tlw = 0.15f * h / sigw * (1.0f - fpm_expf(-5.0f * zeta));
+    } else {
+        sigw = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
+        dsigwdz = -1.3f * ust / h;
+        tlw = 0.1f * h / sigw * fpm_powf(zeta, 0.8f);
+    }
+    tlw = fmaxf(tlw, TLW_MIN);
+    if (dsigwdz == 0.0f) dsigwdz = 1.0e-10f;
+    if (!finite_f(sigw) || sigw <= 0.0f) sigw = SIGMA_FLOOR;
+    if (!finite_f(tlw)) tlw = TLW_MIN;
+    if (!finite_f(dsigwdz)) dsigwdz = 1.0e-10f;
+}
+
+// compute_hanna (langevin_fused.wgsl:224-293): sigma_u,v,w, d sigma_w/dz
+// and Lagrangian time scales at height z.
+__device__ __forceinline__ Turb hanna_full(float ust_in, float wst_in, float ol, float h_in, float z_in)
+{
+    Turb t;
+    t.ust = fmaxf(ust_in, USTAR_MIN);
+    t.wst = fmaxf(wst_in, 0.0f);
+    t.h = fmaxf(h_in, H_MIN);
+    t.ol = ol;
+    const float ust = t.ust, wst = t.wst, h = t.h;
+    const float z = clampf(z_in, 0.0f, h);
+    const float zeta = clampf(z / h, 0.0f, 1.0f);
+    float sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
+    if (neutral_regime(ol, h)) {
+        const float corr = z / ust;
+        sigu = SIGMA_FLOOR + 2.0f * ust * fpm_expf(-3.0e-4f * corr);
+        sigw = 1.3f * ust * fpm_expf(-2.0e-4f * corr);
+        dsigwdz = -2.0e-4f * sigw;
+        sigw = sigw + SIGMA_FLOOR;
+        sigv = sigw;
+        tlu = 0.5f * z / sigw / (1.0f + 1.5e-3f * corr);
+        tlv = tlu;
+        tlw = tlu;
+    } else if (ol < 0.0f) {
+        sigu = SIGMA_FLOOR + ust * fpm_powf(12.0f - 0.5f * h / ol, 1.0f / 3.0f);
+        sigv = sigu;
+        const float p23 = fpm_powf(zeta, 2.0f / 3.0f);
+        const float sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * p23 + (1.8f - 1.4f * zeta) * ust * ust;
+        sigw = fpm_sqrtf(sq) + SIGMA_FLOOR;
+        dsigwdz = 0.5f / sigw / h *
+                  (-1.4f * ust * ust +
+                   wst * wst * (0.8f * fpm_powf(fmaxf(zeta, 1.0e-3f), -1.0f / 3.0f) - 1.8f * p23));
+        tlu = 0.15f * h / sigu;
+        tlv = tlu;
+        if (z < fabsf(ol)) tlw = 0.1f * z / (sigw * (0.55f - 0.38f * fabsf(z / ol)));
+        else if (zeta < 0.1f) tlw = 0.59f * z / sigw;
+        else tlw = 0.15f * h / sigw * (1.0f - fpm_expf(-5.0f * zeta));
+    } else {
+        sigu = SIGMA_FLOOR + 2.0f * ust * (1.0f - zeta);
+        sigv = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
+        sigw = sigv;
+        dsigwdz = -1.3f * ust / h;
+        tlu = 0.15f * h / sigu * fpm_sqrtf(zeta);
+        tlv = 0.467f * tlu;
+        tlw = 0.1f * h / sigw * fpm_powf(zeta, 0.8f);
+    }
+    tlu = fmaxf(tlu, TLU_MIN);
+    tlv = fmaxf(tlv, TLV_MIN);
+    tlw = fmaxf(tlw, TLW_MIN);
+    if (dsigwdz == 0.0f) dsigwdz = 1.0e-10f;
+    if (!finite_f(sigu) || sigu <= 0.0f) sigu = SIGMA_FLOOR;
+    if (!finite_f(sigv) || sigv <= 0.0f) sigv = SIGMA_FLOOR;
+    if (!finite_f(sigw) || sigw <= 0.0f) sigw = SIGMA_FLOOR;
+    if (!finite_f(tlu)) tlu = TLU_MIN;
+    if (!finite_f(tlv)) tlv = TLV_MIN;
+    if (!finite_f(tlw)) tlw = TLW_MIN;
+    if (!finite_f(dsigwdz)) dsigwdz = 1.0e-10f;
+    t.sigu = sigu; t.sigv = sigv; t.sigw = sigw; t.dsigwdz = dsigwdz;
+    t.tlu = tlu; t.tlv = tlv; t.tlw = tlw;
+    return t;
+}
+
+// ---------------------------------------------------------------- Langevin
+
+__device__ __forceinline__ float ou_horizontal(float prev, float sigma_in, float tl_in, float dt, float eta)
+{
+    const float sigma = pos_or_zero(sigma_in);
+    const float tl = pos_or_zero(tl_in);
+    if (tl == 0.0f) return 0.0f;
+    const float r = dt / tl;
+    float nx;
+    if (r < 0.5f) {
+        nx = (1.0f - r) * prev + eta * sigma * fpm_sqrtf(fmaxf(2.0f * r, 0.0f));
+    } else {
+        const float e = fpm_expf(-r);
+        nx = e * prev + eta * sigma * fpm_sqrtf(fmaxf(1.0f - e * e, 0.0f));
+    }
+    return finite_f(nx) ? nx : 0.0f;
+}
+
+__device__ __forceinline__ float ou_vertical(float prev, float sigw_in, float dsigwdz, float tlw_in,
+                                             float dt, float rho, float eta)
+{
+    const float sw = pos_or_zero(sigw_in);
+    const float tl = pos_or_zero(tlw_in);
+    if (tl == 0.0f) return 0.0f;
+    const float drift = sw * (dsigwdz + rho * sw);
+    const float r = dt / tl;
+    float nx;
+    if (r < 0.5f) {
+        nx = (1.0f - r) * prev + eta * sw * fpm_sqrtf(fmaxf(2.0f * r, 0.0f)) + dt * drift;
+    } else {
+        const float e = fpm_expf(-r);
+        nx = e * prev + eta * sw * fpm_sqrtf(fmaxf(1.0f - e * e, 0.0f)) + tl * (1.0f - e) * drift;
+    }
+    return finite_f(nx) ? nx : 0.0f;
+}
+
+// ---------------------------------------------------------------- gridding
+
+__device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, int32_t cx, int32_t cy, float px,
+                                              float py, float z)
+{
+    int32_t fx = (int32_t)floorf(px * (float)g.refine_x);
+    int32_t fy = (int32_t)floorf(py * (float)g.refine_y);
+    fx = max(0, min(fx, (int32_t)g.refine_x - 1));
+    fy = max(0, min(fy, (int32_t)g.refine_y - 1));
+    int64_t ix = (int64_t)cx * g.refine_x + fx;
+    int64_t iy = (int64_t)cy * g.refine_y + fy;
+    ix = ix < 0 ? 0 : (ix > (int64_t)g.nx - 1 ? (int64_t)g.nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > (int64_t)g.ny - 1 ? (int64_t)g.ny - 1 : iy);
+    uint32_t iz;
+    if (g.metres) {
+        iz = g.nz - 1u;
+        for (uint32_t k = 0; k < g.nz; ++k) {
+            if (z <= g.outheights[k]) { iz = k; break; }
+        }
+    } else {
+        int32_t l = (int32_t)floorf(z);
+        iz = (uint32_t)max(0, min(l, (int32_t)g.nz - 1));
+    }
+    return ((uint32_t)ix * g.ny + (uint32_t)iy) * g.nz + iz;
+}
+
+__device__ __forceinline__ uint32_t quantize_mass(float m_in, float scale)
+{
+    const float m = fmaxf(m_in, 0.0f);
+    if (m <= 0.0f) return 0u;
+    const float s = m * scale + 0.5f;
+    return s >= 4294967295.0f ? 0xffffffffu : (uint32_t)s;
+}
+
+}  // namespace
+
+// Wave-level pre-reduction of same-cell contributions followed by one pair of
+// global atomics per distinct cell in the wave.  Every lane of the wave must
+// call this (lanes without a contribution pass valid = false).  Integer sums
+// wrap modulo 2^32 exactly as the reference's atomicAdd does, so the result
+// does not depend on the order of reduction.
+__device__ __forceinline__ void grid_reduce_wave(const GridArgs &g, bool valid, uint32_t cell, uint32_t q)
+{
+    uint64_t pending = __ballot(valid);
+    const uint32_t lane = __lane_id();
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lc = __shfl(cell, (int)leader);
+        const bool mine = valid && cell == lc;
+        const uint64_t same = __ballot(mine);
+        uint32_t v = mine ? q : 0u;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == leader) {
+            atomicAdd(&g.counts[lc], (uint32_t)__popcll(same));
+            atomicAdd(&g.mass_fp[lc], v);
+        }
+        pending &= ~same;
+    }
+}
+
+template <uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
+{
+    __shared__ float lh[kMaxLevels];
+    for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
+    __syncthreads();
+
+    // XCD-aware chunking: the hardware deals blocks round-robin over the 8
+    // XCDs; give each XCD a contiguous run of particle chunks so that
+    // Morton-neighbouring particles share one L2.
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b % kXcds, k = b / kXcds, per = nb / kXcds, rem = nb % kXcds;
+    const uint32_t chunk = xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+    const uint64_t first = (uint64_t)chunk * BLOCK * a.ppt;
+
+    const Soa s = a.s;
+    for (uint32_t r = 0; r < a.ppt; ++r) {
+        const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
+        bool live = i < a.n;
+        uint32_t flags = 0;
+        if (live) flags = s.u(F_FLAGS)[i];
+        live = live && (flags & FLAG_ACTIVE);
+        bool grid_valid = false;
+        uint32_t grid_c = 0, grid_q = 0;
+        if (live) {
+            int32_t cx = s.i(F_CELL_X)[i], cy = s.i(F_CELL_Y)[i];
+            float px = s.f(F_POS_X)[i], py = s.f(F_POS_Y)[i], pz = s.f(F_POS_Z)[i];
+            float tu = s.f(F_TURB_U)[i], tv = s.f(F_TURB_V)[i], tw = s.f(F_TURB_W)[i];
+            int32_t cbt = s.i(F_CBT)[i];
+            float mass[FP_MAX_SPECIES];
+#pragma unroll
+            for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
+                if (q < a.n_species) mass[q] = s.f(F_MASS0 + q)[i];
+
+            // ---- advection (advection_dual_wind.wgsl:254-303)
+            const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
+            const float x0 = (float)cx + px, y0 = (float)cy + py, z0 = pz;
+            const Wind s0 = sample_wind(a, lh, x0, y0, z0);
+            const float u0 = s0.u + tu, v0 = s0.v + tv, w0 = s0.w;
+            const float qx = clampf(x0 + (u0 * a.x_scale) * a.dt_adv, 0.0f, nxm);
+            const float qy = clampf(y0 + (v0 * a.y_scale) * a.dt_adv, 0.0f, nym);
+            const float qz = clampf(z0 + (w0 * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
+            const Wind s1 = sample_wind(a, lh, qx, qy, qz);
+            const float u1 = s1.u + tu, v1 = s1.v + tv, w1 = s1.w;
+            const float cu = 0.5f * (u0 + u1), cv = 0.5f * (v0 + v1), cw = 0.5f * (w0 + w1);
+            const float x1 = clampf(x0 + (cu * a.x_scale) * a.dt_adv, 0.0f, nxm);
+            const float y1 = clampf(y0 + (cv * a.y_scale) * a.dt_adv, 0.0f, nym);
+            pz = clampf(z0 + (cw * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
+            split_cell(x1, a.nx, cx, px);
+            split_cell(y1, a.ny, cy, py);
+
+            // ---- PBL lookup + Hanna (langevin_fused.wgsl:414-427)
+            const float hx = (float)cx + px, hy = (float)cy + py;
+            const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
+            const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
+            const uint32_t pidx = pi * a.ny + pj;
+            const float4 pb = a.pbl[pidx];
+            const float ol = fabsf(pb.w) < 1.0e-5f ? __builtin_inff() : 1.0f / pb.w;
+            Turb t = hanna_full(pb.x, pb.y, ol, pb.z, pz);
+
+            // ---- Philox blocks per slot (langevin_fused.wgsl:430-451)
+            const uint64_t cbase = i * a.blocks_per_particle;
+            const U4 b0 = philox10(ctr_add(a.c0, a.c1, a.c2, a.c3, cbase), a.key0, a.key1);
+            U4 b1 = U4{0u, 0u, 0u, 0u};
+            if (a.blocks_per_particle > 1u)
+                b1 = philox10(ctr_add(a.c0, a.c1, a.c2, a.c3, cbase + 1u), a.key0, a.key1);
+            float eu, ev, ew0, ew1, ew2, ew3;
+            gauss_pair(b0.x, b0.y, eu, ev);
+            gauss_pair(b0.z, b0.w, ew0, ew1);
+            gauss_pair(b1.x, b1.y, ew2, ew3);
+
+            // ---- horizontal OU over the full step
+            tu = ou_horizontal(tu, t.sigu, t.tlu, a.dt, eu);
+            tv = ou_horizontal(tv, t.sigv, t.tlv, a.dt, ev);
+
+            // ---- vertical OU with sub-steps and reflection
+            if (a.n_substeps == 0u) {
+                tw = ou_vertical(tw, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0);
+            } else {
+                const float dts = a.dt / (float)a.n_substeps;
+                const float zmin = a.min_height;
+                float sigw = t.sigw, dsw = t.dsigwdz, tlw = t.tlw;
+                for (uint32_t sub = 0; sub < a.n_substeps; ++sub) {
+                    const float eta = sub == 0u ? ew0 : (sub == 1u ? ew1 : (sub == 2u ? ew2 : ew3));
+                    tw = ou_vertical(tw, sigw, dsw, tlw, dts, a.rho_grad, eta);
+                    pz = pz + tw * dts;
+                    if (t.h > zmin) {
+                        float z = pz;
+                        for (int bounce = 0; bounce < 10; ++bounce) {
+                            if (z < zmin) { z = 2.0f * zmin - z; cbt = -cbt; tw = -tw; }
+                            else if (z > t.h) { z = 2.0f * t.h - z; cbt = -cbt; tw = -tw; }
+                            else break;
+                        }
+                        pz = clampf(z, zmin, t.h);
+                    }
+                    if (sub + 1u < a.n_substeps) {
+                        // hanna_short_w at the new height (langevin_fused.wgsl:502-513)
+                        const float zz = clampf(pz, 0.0f, t.h);
+                        const float zeta = clampf(zz / t.h, 0.0f, 1.0f);
+                        vertical_profile(t.ust, t.wst, t.ol, t.h, zz, zeta, sigw, dsw, tlw);
+                    }
+                }
+            }
+
+            // ---- dry deposition (dry_deposition.wgsl:70-106)
+            if (a.dry_mode != FP_DRY_OFF) {
+                const float href = fmaxf(a.dry_href, 0.1f);
+                if (!(pz >= 2.0f * href)) {
+                    const float vd = a.dry_mode == FP_DRY_PER_SLOT ? a.dry_vd_slots[i] : a.dry_vd;
+                    const float ex = -fmaxf(vd, 0.0f) * fabsf(a.dt) / (2.0f * href);
+                    const float sv = clampf(fpm_expf(ex), 0.0f, 1.0f);
+#pragma unroll
+                    for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
+                        if (q < a.n_species) mass[q] = mass[q] * sv;
+                }
+            }
+            // ---- wet deposition (wet_deposition.wgsl:73-111)
+            if (a.wet_mode != FP_WET_OFF) {
+                float lam, frac;
+                if (a.wet_mode == FP_WET_PER_SLOT) {
+                    lam = a.wet_lambda_slots[i];
+                    frac = a.wet_fraction_slots[i];
+                } else if (a.wet_mode == FP_WET_PRECIP_FIELD) {
+                    // below_cloud_scavenging_coefficient_gas (physics/wet_scavenging.rs:213-225)
+                    const float pr = pos_or_zero(a.precip[pidx]);
+                    lam = 0.0f;
+                    if (pr > 0.0f) lam = pos_or_zero(pos_or_zero(a.wet_a) * fpm_powf(pr, pos_or_zero(a.wet_b)));
+                    frac = a.wet_fraction;
+                } else {
+                    lam = a.wet_lambda;
+                    frac = a.wet_fraction;
+                }
+                const float l = fmaxf(lam, 0.0f);
+                const float dt = fabsf(a.dt);
+                const float fr = clampf(frac, 0.0f, 1.0f);
+                float prob = 0.0f;
+                if (!(l <= 0.0f || dt <= 0.0f || fr <= 0.0f))
+                    prob = clampf((1.0f - fpm_expf(-l * dt)) * fr, 0.0f, 1.0f);
+                const float sv = 1.0f - prob;
+#pragma unroll
+                for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
+                    if (q < a.n_species) mass[q] = mass[q] * sv;
+            }
+            if (a.loss_mode == FP_LOSS_MIN_MASS && mass[0] < a.loss_min_mass) flags &= ~FLAG_ACTIVE;
+
+            // ---- store
+            s.i(F_CELL_X)[i] = cx; s.i(F_CELL_Y)[i] = cy;
+            s.f(F_POS_X)[i] = px; s.f(F_POS_Y)[i] = py; s.f(F_POS_Z)[i] = pz;
+            s.f(F_TURB_U)[i] = tu; s.f(F_TURB_V)[i] = tv; s.f(F_TURB_W)[i] = tw;
+            s.f(F_VEL_U)[i] = cu; s.f(F_VEL_V)[i] = cv; s.f(F_VEL_W)[i] = cw;
+            s.i(F_CBT)[i] = cbt;
+            s.u(F_FLAGS)[i] = flags;
+#pragma unroll
+            for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
+                if (q < a.n_species) s.f(F_MASS0 + q)[i] = mass[q];
+
+            if (a.grid_fused && (flags & FLAG_ACTIVE)) {
+                grid_valid = true;
+                grid_c = grid_cell(a.grid, cx, cy, px, py, pz);
+                grid_q = quantize_mass(mass[a.grid.species], a.grid.mass_scale);
+            }
+        }
+        if (a.grid_fused) grid_reduce_wave(a.grid, grid_valid, grid_c, grid_q);
+    }
+}
+
+template <uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs g)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    bool valid = false;
+    uint32_t cell = 0, q = 0;
+    if (i < n && (s.u(F_FLAGS)[i] & FLAG_ACTIVE)) {
+        valid = true;
+        cell = grid_cell(g, s.i(F_CELL_X)[i], s.i(F_CELL_Y)[i], s.f(F_POS_X)[i], s.f(F_POS_Y)[i],
+                         s.f(F_POS_Z)[i]);
+        q = quantize_mass(s.f(F_MASS0 + g.species)[i], g.mass_scale);
+    }
+    grid_reduce_wave(g, valid, cell, q);
+}
+
+hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st)
+{
+    if (a.n == 0) return hipSuccess;
+    const uint64_t per_block = (uint64_t)block * a.ppt;
+    const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
+    switch (block) {
+    case 64: step_kernel<64><<<grid, 64, 0, st>>>(a); break;
+    case 128: step_kernel<128><<<grid, 128, 0, st>>>(a); break;
+    case 256: step_kernel<256><<<grid, 256, 0, st>>>(a); break;
+    case 512: step_kernel<512><<<grid, 512, 0, st>>>(a); break;
+    case 1024: step_kernel<1024><<<grid, 1024, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grid(const Soa &s, uint32_t n, const GridArgs &g, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (n + 255u) / 256u;
+    grid_kernel<256><<<grid, 256, 0, st>>>(s, n, g);
+    return hipGetLastError();
+}
+
+}  // namespace fp

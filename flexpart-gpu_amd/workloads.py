@@ -1,0 +1,248 @@
+"""Deterministic synthetic inputs for the per-particle timestep.
+
+The reference builds its test and benchmark met fields synthetically
+(wind/synthetic.rs: uniform_wind_field, linear_shear_wind_field) and its
+benchmark scenario in src/bin/fortran-validation.rs.  This module generates
+the two workloads the timestep is measured and checked on, from seeds only,
+so that the GPU run and the CPU oracle see identical inputs:
+
+* ``point_release`` - 100,000 particles, one point release at the grid centre
+  100 m above ground emitting 1 kg uniformly over the first hour, 24 h at a
+  900 s step, a 64 x 64 (1 deg) x 32-level met grid with u = 10 m/s +
+  2e-3 s^-1 * z, v = 2 m/s, w = 0, PBL height 1000 m, u* 0.4 m/s,
+  w* 1.5 m/s, L = -200 m, dry deposition v_d = 2e-3 m/s, Philox seed 42,
+  output 64 x 64 with levels 100/500/1000/3000 m.
+
+* ``global_10m`` - 10,000,000 particles from 16 point sources at
+  uniform-random positions with heights U[10, 500] m and lognormal emissions
+  (median 1 kg, sigma 1), 24 h at a 600 s step on a 360 x 181 (1 deg global)
+  x 60-level met grid with time levels 6 h apart; winds are a zonal jet
+  u = 30 sin(2 lat) m/s plus random Fourier modes (wavenumbers <= 8, rms 5
+  m/s), w from continuity; hmix lognormal (median 800 m, sigma 0.5), 1/L
+  U[-0.02, 0.02] 1/m, u* U[0.1, 0.8] m/s; output 720 x 361 x 5 levels.
+
+Level heights follow a stretched table z_k = z_top * (k/(nz-1))^2 + 10 m,
+(strictly increasing, 10 m at the surface level).  w* is 1.5 m/s *
+(hmix/1000 m)^(1/3) where 1/L < 0 (convective) and 0 elsewhere.  The
+particles of a source are released at t = 0 at the source point (the release
+manager's time window is host-side work outside the timestep).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+EARTH_RADIUS_M = 6.371e6
+METRES_PER_DEG = EARTH_RADIUS_M * math.pi / 180.0
+
+
+def level_heights(nz: int, z_top: float) -> np.ndarray:
+    k = np.arange(nz, dtype=np.float64)
+    return (z_top * (k / (nz - 1)) ** 2 + 10.0).astype(np.float32)
+
+
+@dataclass
+class Met:
+    shape: tuple
+    levels: list            # list of (u, v, w) float32 arrays, flat ((ix*ny)+iy)*nz+iz
+    level_heights: np.ndarray
+    seconds_between_levels: float
+
+
+@dataclass
+class Pbl:
+    shape: tuple
+    ustar: np.ndarray
+    wstar: np.ndarray
+    hmix: np.ndarray
+    oli: np.ndarray
+    precip: np.ndarray | None = None
+
+
+@dataclass
+class Workload:
+    name: str
+    met: Met
+    pbl: Pbl
+    particles: np.ndarray     # PARTICLE_DTYPE, all slots (capacity)
+    dt_seconds: float
+    n_steps: int
+    x_scale: float
+    y_scale: float
+    key: tuple
+    out_shape: tuple
+    outheights: tuple
+    refine: tuple
+    mass_scale: float
+    dry_vd: float = 0.0
+    release_seconds: float = 0.0  # emit uniformly over [0, release_seconds]
+    extra: dict = field(default_factory=dict)
+
+    def alpha(self, t_seconds: float) -> tuple:
+        """(bracket index, alpha) for simulation time t."""
+        dtm = self.met.seconds_between_levels
+        b = min(int(t_seconds // dtm), len(self.met.levels) - 2)
+        return b, float((t_seconds - b * dtm) / dtm)
+
+
+def make_particles(n: int, x, y, z, mass, capacity: int | None = None) -> np.ndarray:
+    """Active particles at grid coordinates (x, y) [grid units], z [m]."""
+    from . import PARTICLE_DTYPE as dt
+    cap = n if capacity is None else capacity
+    p = np.zeros(cap, dt)
+    x = np.broadcast_to(np.asarray(x, np.float32), (n,))
+    y = np.broadcast_to(np.asarray(y, np.float32), (n,))
+    cx = np.floor(x).astype(np.int32)
+    cy = np.floor(y).astype(np.int32)
+    p["cell_x"][:n] = cx
+    p["cell_y"][:n] = cy
+    p["pos_x"][:n] = (x - cx).astype(np.float32)
+    p["pos_y"][:n] = (y - cy).astype(np.float32)
+    p["pos_z"][:n] = np.broadcast_to(np.asarray(z, np.float32), (n,))
+    p["mass"][:n, 0] = np.broadcast_to(np.asarray(mass, np.float32), (n,))
+    p["flags"][:n] = 1
+    p["cbt"][:n] = 1
+    return p
+
+
+def point_release(n_particles: int = 100_000, seed: int = 42) -> Workload:
+    nx, ny, nz = 64, 64, 32
+    lh = level_heights(nz, 20_000.0)
+    z = np.broadcast_to(lh[None, None, :], (nx, ny, nz))
+    u = (10.0 + 2.0e-3 * z).astype(np.float32).reshape(-1)
+    v = np.full(nx * ny * nz, 2.0, np.float32)
+    w = np.zeros(nx * ny * nz, np.float32)
+    met = Met((nx, ny, nz), [(u, v, w), (u.copy(), v.copy(), w.copy())], lh, 3 * 3600.0)
+    np2 = nx * ny
+    pbl = Pbl((nx, ny), np.full(np2, 0.4, np.float32), np.full(np2, 1.5, np.float32),
+              np.full(np2, 1000.0, np.float32), np.full(np2, -1.0 / 200.0, np.float32))
+    parts = make_particles(n_particles, nx / 2.0, ny / 2.0, 100.0, 1.0 / n_particles)
+    parts["flags"][:] = 0  # released over the first hour by the caller
+    scale = 1.0 / METRES_PER_DEG
+    return Workload("point_release", met, pbl, parts, 900.0, 96, scale, scale, (seed, 0),
+                    (64, 64, 4), (100.0, 500.0, 1000.0, 3000.0), (1, 1), 1.0e9,
+                    dry_vd=2.0e-3, release_seconds=3600.0)
+
+
+def _fourier_field(rng, nx, ny, kmax, rms):
+    lon = np.deg2rad(np.arange(nx, dtype=np.float64))
+    lat = np.deg2rad(np.arange(ny, dtype=np.float64) - 90.0)
+    L, P = np.meshgrid(lon, lat, indexing="ij")
+    f = np.zeros((nx, ny))
+    for kx in range(0, kmax + 1):
+        for ky in range(0, kmax + 1):
+            if kx == 0 and ky == 0:
+                continue
+            a = rng.normal()
+            ph = rng.uniform(0, 2 * np.pi)
+            f += a * np.cos(kx * L + 2 * ky * P + ph)
+    f *= rms / max(np.sqrt(np.mean(f ** 2)), 1e-12)
+    return f
+
+
+def _wind_level(rng, nx, ny, lh):
+    lat = np.deg2rad(np.arange(ny, dtype=np.float64) - 90.0)
+    up = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0))
+    vp = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0))
+    u2 = 30.0 * np.sin(2.0 * lat)[None, :] + up
+    v2 = vp
+    # horizontal divergence on the sphere (centred differences, periodic in lon)
+    coslat = np.maximum(np.cos(lat), 1e-3)[None, :]
+    dx = METRES_PER_DEG * coslat
+    dudx = (np.roll(u2, -1, 0) - np.roll(u2, 1, 0)) / (2.0 * dx)
+    dvdy = np.gradient(v2 * coslat, axis=1) / (METRES_PER_DEG * coslat)
+    div = dudx + dvdy
+    zf = lh.astype(np.float64)
+    g = 0.6 + 0.4 * np.minimum(zf / 10_000.0, 1.0)           # wind profile factor
+    G = np.concatenate([[0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(zf))])  # int_0^z g
+    u = (u2[:, :, None] * g[None, None, :]).astype(np.float32)
+    v = (v2[:, :, None] * g[None, None, :]).astype(np.float32)
+    w = (-div[:, :, None] * G[None, None, :]).astype(np.float32)  # continuity
+    return u.reshape(-1), v.reshape(-1), w.reshape(-1)
+
+
+def global_met(seed: int = 2, nx: int = 360, ny: int = 181, nz: int = 60,
+               n_levels: int = 5) -> Met:
+    lh = level_heights(nz, 25_000.0)
+    levels = [_wind_level(np.random.default_rng(seed * 1000 + t), nx, ny, lh)
+              for t in range(n_levels)]
+    return Met((nx, ny, nz), levels, lh, 6 * 3600.0)
+
+
+def global_pbl(seed: int = 2, nx: int = 360, ny: int = 181) -> Pbl:
+    rng = np.random.default_rng(seed * 7919)
+    n = nx * ny
+    hmix = (800.0 * np.exp(0.5 * rng.normal(size=n))).astype(np.float32)
+    oli = rng.uniform(-0.02, 0.02, n).astype(np.float32)
+    ustar = rng.uniform(0.1, 0.8, n).astype(np.float32)
+    wstar = np.where(oli < 0, 1.5 * np.cbrt(hmix / 1000.0), 0.0).astype(np.float32)
+    return Pbl((nx, ny), ustar, wstar, hmix, oli, np.zeros(n, np.float32))
+
+
+def global_sources(n_particles: int, seed: int = 2, n_sources: int = 16, nx=360, ny=181):
+    rng = np.random.default_rng(seed * 104729)
+    sx = rng.uniform(0.0, nx - 1.0, n_sources)
+    sy = rng.uniform(10.0, ny - 11.0, n_sources)
+    sz = rng.uniform(10.0, 500.0, n_sources)
+    emis = np.exp(rng.normal(size=n_sources))  # lognormal, median 1 kg, sigma 1
+    per = np.full(n_sources, n_particles // n_sources)
+    per[: n_particles % n_sources] += 1
+    src = np.repeat(np.arange(n_sources), per)
+    x = sx[src].astype(np.float32)
+    y = sy[src].astype(np.float32)
+    z = sz[src].astype(np.float32)
+    mass = (emis[src] / per[src]).astype(np.float32)
+    return x, y, z, mass
+
+
+def global_10m(n_particles: int = 10_000_000, seed: int = 2, met: Met | None = None,
+               pbl: Pbl | None = None) -> Workload:
+    met = global_met(seed) if met is None else met
+    pbl = global_pbl(seed) if pbl is None else pbl
+    x, y, z, mass = global_sources(n_particles, seed)
+    parts = make_particles(n_particles, x, y, z, mass)
+    scale = 1.0 / METRES_PER_DEG
+    return Workload("global_10m", met, pbl, parts, 600.0, 144, scale, scale, (seed, 0),
+                    (720, 361, 5), (100.0, 500.0, 1000.0, 3000.0, 10_000.0), (2, 2), 1.0e8)
+
+
+def small_random(n_particles: int, shape=(64, 64, 32), seed: int = 7, n_levels: int = 2):
+    """Small randomized workload for parity tests: random Fourier winds on a
+    reduced grid, random PBL fields covering all three Hanna regimes and
+    particles spread over the domain and the boundary layer."""
+    nx, ny, nz = shape
+    met_rng = np.random.default_rng(seed)
+    lh = level_heights(nz, 15_000.0)
+    levels = []
+    for _ in range(n_levels):
+        u = met_rng.normal(5.0, 8.0, (nx, ny, nz)).astype(np.float32)
+        v = met_rng.normal(0.0, 6.0, (nx, ny, nz)).astype(np.float32)
+        w = met_rng.normal(0.0, 0.05, (nx, ny, nz)).astype(np.float32)
+        levels.append((u.reshape(-1), v.reshape(-1), w.reshape(-1)))
+    met = Met(shape, levels, lh, 3 * 3600.0)
+    n = nx * ny
+    hmix = (800.0 * np.exp(0.5 * met_rng.normal(size=n))).astype(np.float32)
+    oli = met_rng.uniform(-0.02, 0.02, n).astype(np.float32)
+    oli[met_rng.uniform(size=n) < 0.1] = 0.0  # neutral cells
+    ustar = met_rng.uniform(0.1, 0.8, n).astype(np.float32)
+    wstar = np.where(oli < 0, 1.5 * np.cbrt(hmix / 1000.0), 0.0).astype(np.float32)
+    precip = np.where(met_rng.uniform(size=n) < 0.3,
+                      2.0 * np.exp(met_rng.normal(size=n)), 0.0).astype(np.float32)
+    pbl = Pbl((nx, ny), ustar, wstar, hmix, oli, precip)
+    rng = np.random.default_rng(seed + 1)
+    x = rng.uniform(0, nx - 1, n_particles).astype(np.float32)
+    y = rng.uniform(0, ny - 1, n_particles).astype(np.float32)
+    z = rng.uniform(0.0, 2500.0, n_particles).astype(np.float32)
+    mass = np.exp(rng.normal(size=n_particles)).astype(np.float32)
+    parts = make_particles(n_particles, x, y, z, mass)
+    parts["turb_u"] = rng.normal(0, 0.5, n_particles)
+    parts["turb_v"] = rng.normal(0, 0.5, n_particles)
+    parts["turb_w"] = rng.normal(0, 0.3, n_particles)
+    parts["cbt"] = np.where(rng.uniform(size=n_particles) < 0.5, 1, -1)
+    parts["flags"][rng.uniform(size=n_particles) < 0.1] = 0  # inactive slots
+    scale = 1.0 / 25_000.0
+    return Workload("small_random", met, pbl, parts, 600.0, 4, scale, scale, (0xDECAFBAD, 0x12345678),
+                    (shape[0] * 2, shape[1] * 2, 4), (100.0, 500.0, 1000.0, 3000.0), (2, 2), 1.0e6,
+                    dry_vd=2.0e-3)

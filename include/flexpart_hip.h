@@ -1,0 +1,237 @@
+/*
+ * flexpart_hip.h - C ABI of the MI355X (gfx950) per-particle timestep.
+ *
+ * This library replaces the reference's GPU backend (src/gpu + src/shaders of
+ * the flexpart-gpu crate) behind plain C calls: the crate's time loop
+ * (simulation/timeloop.rs) keeps its control flow and calls these entry
+ * points where it encodes wgpu passes today.  Every entry point names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *   - Status: 0 = OK, negative = error class (FP_E_*); fp_last_error() holds
+ *     the message.  Nothing unwinds across the ABI.
+ *   - A context owns one HIP stream and every device buffer (RAII in the
+ *     reference: the driver struct owns every wgpu buffer, timeloop.rs:713-828).
+ *     Calls on one context are stream-ordered and not thread-safe.
+ *     fp_sync() and calls that return host data block; the rest are async.
+ *   - Host pointers are plain host memory; "_dev" entry points take device
+ *     pointers already resident in HBM.
+ *   - Particle state is stored as SoA in HBM.  The AoS exchange format is the
+ *     reference's 96-byte Particle (particles/mod.rs:76-168).
+ */
+#ifndef FLEXPART_HIP_H
+#define FLEXPART_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_OK 0
+#define FP_E_INVALID (-1)   /* bad argument / shape */
+#define FP_E_DEVICE (-2)    /* HIP runtime error */
+#define FP_E_NOMEM (-3)     /* device allocation failed */
+#define FP_E_STATE (-4)     /* call order (e.g. step before met upload) */
+
+#define FP_FLAG_ACTIVE 1u
+#define FP_MAX_SPECIES 4
+#define FP_MAX_OUTPUT_LEVELS 16
+
+/* Reference Particle, 96 bytes, repr(C) (particles/mod.rs:76-168). */
+typedef struct fp_particle {
+    float pos_x, pos_y, pos_z;
+    int32_t cell_x, cell_y;
+    uint32_t flags;
+    float mass[FP_MAX_SPECIES];
+    float vel_u, vel_v, vel_w;
+    float turb_u, turb_v, turb_w;
+    int32_t time, timestep, time_mem, time_split, release_point, class_id, cbt;
+    uint32_t pad0;
+} fp_particle;
+
+typedef struct fp_ctx fp_ctx;
+
+/* Context shape.  met_* is the wind grid (WindField3D shape, wind/mod.rs:138),
+ * the PBL grid is met_nx x met_ny (PblState, pbl/mod.rs:130-174). */
+typedef struct fp_ctx_desc {
+    uint64_t capacity;      /* particle slots (ParticleStore::with_capacity) */
+    uint32_t met_nx, met_ny, met_nz;
+    uint32_t n_species;     /* mass slots the step updates, 1..4 */
+    uint32_t out_nx, out_ny, out_nz; /* concentration grid (0 = none) */
+    uint32_t reserved[6];
+} fp_ctx_desc;
+
+/* Per-step parameters.  Gathers DualWindAdvectionParams (gpu/advection.rs:
+ * 58-73), FusedLangevinParams (gpu/langevin_fused.rs:98-115) and the
+ * deposition dispatch params (gpu/deposition.rs, gpu/wet_deposition.rs). */
+typedef struct fp_step_params {
+    float dt_seconds;        /* > 0 */
+    float alpha;             /* temporal weight, clamped to [0,1] */
+    float x_scale, y_scale, z_scale; /* VelocityToGridScale */
+    int32_t direction;       /* +1 forward, -1 backward (advection dt sign) */
+    uint32_t key[2];         /* Philox key */
+    uint32_t counter[4];     /* Philox base counter for this step */
+    uint32_t n_substeps;     /* vertical sub-steps 0..4 (timeloop.rs:1111: 4) */
+    float min_height_m;      /* reflection floor (timeloop.rs:1112: 0.01) */
+    float rho_grad_over_rho;
+    uint32_t dry_mode;       /* FP_DRY_* */
+    float dry_vd;            /* m/s, FP_DRY_UNIFORM */
+    float dry_href;          /* reference height (constants::HREF = 15 m) */
+    uint32_t wet_mode;       /* FP_WET_* */
+    float wet_lambda;        /* 1/s, FP_WET_UNIFORM */
+    float wet_fraction;      /* FP_WET_UNIFORM and FP_WET_PRECIP_FIELD */
+    float wet_a, wet_b;      /* lambda = a * P^b, FP_WET_PRECIP_FIELD */
+    uint32_t loss_mode;      /* FP_LOSS_* */
+    float loss_min_mass;     /* kg, FP_LOSS_MIN_MASS */
+    uint32_t grid_fused;     /* 1: accumulate the concentration grid after the step */
+    uint32_t reserved[7];
+} fp_step_params;
+
+#define FP_DRY_OFF 0u
+#define FP_DRY_UNIFORM 1u
+#define FP_DRY_PER_SLOT 2u   /* v_d[slot] from fp_forcing_upload */
+#define FP_WET_OFF 0u
+#define FP_WET_UNIFORM 1u
+#define FP_WET_PER_SLOT 2u   /* lambda[slot], fraction[slot] */
+#define FP_WET_PRECIP_FIELD 3u /* lambda = a * precip(i,j)^b, nearest PBL cell */
+#define FP_LOSS_NONE 0u
+#define FP_LOSS_MIN_MASS 1u  /* clear FLAG_ACTIVE when mass[0] < loss_min_mass */
+
+/* Concentration gridding (ConcentrationGriddingParams, gpu/gridding.rs:90-110). */
+typedef struct fp_grid_params {
+    uint32_t refine_x, refine_y; /* output cells per met cell; 1 = met grid */
+    uint32_t species;
+    float mass_scale;            /* fixed-point scale (reference default 1e6) */
+    float outheights[FP_MAX_OUTPUT_LEVELS]; /* metres; all zero = level index */
+} fp_grid_params;
+
+/* Morton re-sort (ParticleSpatialSortOptions, particles/mod.rs:321-340). */
+typedef struct fp_sort_params {
+    uint32_t bits_per_axis;      /* 1..21 */
+    uint32_t explicit_bounds;    /* 0: active-particle extent */
+    double bounds[6];            /* x_min,x_max,y_min,y_max,z_min,z_max */
+} fp_sort_params;
+
+/* ---- context -------------------------------------------------------- */
+
+/* Replaces GpuContext::new (gpu/mod.rs:163-208) + ParticleBuffers::from_store
+ * (gpu/buffers.rs:381-420): selects the device, allocates every buffer. */
+int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out);
+int fp_ctx_destroy(fp_ctx *ctx);
+/* Last error message of this context (or of the last failed create). */
+const char *fp_last_error(const fp_ctx *ctx);
+/* Blocks until all work on the context stream is done
+ * (device.poll(Maintain::Wait), timeloop.rs:1051-1058). */
+int fp_sync(fp_ctx *ctx);
+/* Library version string. */
+const char *fp_version(void);
+
+/* ---- particles ------------------------------------------------------ */
+
+/* ParticleBuffers::upload_particle_slots (gpu/buffers.rs:464-493). */
+int fp_particles_upload_slots(fp_ctx *ctx, const uint32_t *slots, const fp_particle *p,
+                              uint64_t n);
+/* ParticleBuffers::upload_store (gpu/buffers.rs:440-462): slots [0, n). */
+int fp_particles_upload(fp_ctx *ctx, const fp_particle *p, uint64_t n);
+/* ParticleBuffers::download_particles (gpu/buffers.rs:111-166): slots [0, n). */
+int fp_particles_download(fp_ctx *ctx, fp_particle *out, uint64_t n);
+/* ParticleBuffers::set_dispatch_count (gpu/buffers.rs:424-436). */
+int fp_particles_set_dispatch_count(fp_ctx *ctx, uint64_t n);
+uint64_t fp_particles_dispatch_count(const fp_ctx *ctx);
+/* Device pointers of the SoA arrays (field order of fp_particle, mass as
+ * FP_MAX_SPECIES arrays); valid until the next compaction or sort. */
+int fp_particles_soa_pointers(fp_ctx *ctx, void **ptrs, uint32_t n_ptrs);
+
+/* ---- met / PBL / forcing -------------------------------------------- */
+
+/* DualWindBuffers::from_fields / upload_t0 / upload_t1 (gpu/buffers.rs:801-953):
+ * six fields of met_nx*met_ny*met_nz floats, flat ((ix*ny)+iy)*nz+iz, and the
+ * level heights [m] (nz entries, non-decreasing; all zero = level-index z,
+ * VelocityToGridScale::IDENTITY). */
+int fp_met_upload_bracket(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                          const float *u1, const float *v1, const float *w1,
+                          const float *level_heights);
+int fp_met_upload_bracket_dev(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                              const float *u1, const float *v1, const float *w1,
+                              const float *level_heights_host);
+/* PblBuffers::upload_state (gpu/pbl.rs): ustar, wstar, hmix, oli (1/L) on
+ * met_nx x met_ny, index i*ny+j; precip [mm/h] may be NULL. */
+int fp_pbl_upload(fp_ctx *ctx, const float *ustar, const float *wstar, const float *hmix,
+                  const float *oli, const float *precip);
+/* DryDepositionIoBuffers / WetDepositionIoBuffers uploads: per-slot forcing
+ * (capacity floats each; NULL leaves that array unchanged). */
+int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,
+                      const float *wet_fraction);
+
+/* ---- the step ------------------------------------------------------- */
+
+/* One production timestep over the dispatch range: the four reference passes
+ * encode_advection_dual_wind_gpu_with_kernel (gpu/advection.rs:605),
+ * encode_langevin_fused_gpu (gpu/langevin_fused.rs:133),
+ * encode_dry_deposition_probability_gpu_with_kernel (gpu/deposition.rs:315),
+ * encode_wet_deposition_probability_gpu_with_kernel (gpu/wet_deposition.rs:368)
+ * fused in one kernel.  next_counter = counter + dispatch*(n_sub<=2 ? 1 : 2)
+ * (gpu/langevin_fused.rs:229-233). */
+int fp_step(fp_ctx *ctx, const fp_step_params *params, uint32_t next_counter[4]);
+
+/* ---- concentration grid --------------------------------------------- */
+
+/* ConcentrationGridBuffers::clear (gpu/gridding.rs:178-188). */
+int fp_grid_clear(fp_ctx *ctx);
+/* Sets the gridding parameters used by fp_grid_accumulate and by fused
+ * gridding in fp_step. */
+int fp_grid_configure(fp_ctx *ctx, const fp_grid_params *params);
+/* dispatch_concentration_gridding_gpu (gpu/gridding.rs:255-407). */
+int fp_grid_accumulate(fp_ctx *ctx);
+/* download_output (gpu/gridding.rs:191-218), raw u32 sums (blocking). */
+int fp_grid_download(fp_ctx *ctx, uint32_t *counts, uint32_t *mass_fp);
+
+/* ---- compaction and Morton re-sort ---------------------------------- */
+
+/* encode_compaction_with_reorder (gpu/compaction.rs:447-505) +
+ * download_active_count (:290) + set_dispatch_count: survivors keep ascending
+ * slot order, the tail holds deactivated copies.  survivor_map (host, may be
+ * NULL) receives new_slot -> old_slot for the active prefix.  Blocking. */
+int fp_compact(fp_ctx *ctx, uint64_t *active_count, uint32_t *survivor_map);
+
+/* ParticleStore::sort_spatially (particles/mod.rs:649-699) on the device:
+ * stable sort of the dispatch range by Morton key, inactive slots last.
+ * new_to_old (host, may be NULL) receives the permutation.  Blocking only
+ * when new_to_old is given. */
+int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *params, uint32_t *new_to_old);
+
+/* ---- RNG known-answer path ------------------------------------------ */
+
+/* philox_rng.wgsl (gpu/rng.rs): n consecutive Philox4x32-10 blocks at
+ * counter + i, written to out[4*i..4*i+3] (host).  Blocking. */
+int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[2],
+                     uint64_t n, uint32_t *out);
+
+/* ---- timing ----------------------------------------------------------- */
+
+/* HIP-event timing of each kernel launch on the context stream (the
+ * reference's FLEXPART_GPU_PROFILE wall-clock sections, timeloop.rs:398-467,
+ * with device timestamps).  kernel ids: FP_KERNEL_*. */
+#define FP_KERNEL_STEP 0
+#define FP_KERNEL_GRID 1
+#define FP_KERNEL_COMPACT 2
+#define FP_KERNEL_SORT 3
+#define FP_KERNEL_COUNT 4
+int fp_profile_enable(fp_ctx *ctx, int enable);
+/* Blocking: total milliseconds and launch count since the last reset. */
+int fp_profile_read(fp_ctx *ctx, int kernel, double *total_ms, uint64_t *launches);
+int fp_profile_reset(fp_ctx *ctx);
+
+/* ---- tuning ----------------------------------------------------------- */
+
+/* Launch shape of the step kernel (gpu/workgroup.rs runtime_workgroup_size):
+ * block size (64..1024, multiple of 64) and particles per thread (1..16). */
+int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_thread);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLEXPART_HIP_H */

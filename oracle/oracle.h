@@ -1,0 +1,111 @@
+/*
+ * oracle.h - CPU restatement of FLEXPART-GPU's production per-particle
+ * timestep.  TEST INFRASTRUCTURE ONLY: this library is the checker for the
+ * HIP path.  Only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark load it; the product never does.
+ *
+ * State is the reference's 96-byte AoS particle (particles/mod.rs:76-168),
+ * met fields use the reference flat layout ((ix*ny)+iy)*nz+iz
+ * (advection_dual_wind.wgsl:82-84) and PBL fields i*ny+j
+ * (langevin_fused.wgsl:209-211).
+ */
+#ifndef FP_ORACLE_H
+#define FP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    float pos_x, pos_y, pos_z;
+    int32_t cell_x, cell_y;
+    uint32_t flags;
+    float mass[4];
+    float vel_u, vel_v, vel_w;
+    float turb_u, turb_v, turb_w;
+    int32_t time, timestep, time_mem, time_split, release_point, class_id, cbt;
+    uint32_t pad0;
+} orc_particle;
+
+typedef struct {
+    uint32_t nx, ny, nz;
+    const float *u0, *v0, *w0, *u1, *v1, *w1;
+    const float *level_heights; /* nz entries; top <= 0 selects level-index z */
+} orc_met;
+
+typedef struct {
+    uint32_t nx, ny;
+    const float *ustar, *wstar, *hmix, *oli;
+    const float *precip; /* mm/h, used by wet_mode 3 only */
+} orc_pbl;
+
+typedef struct {
+    float dt_seconds;
+    float alpha;
+    float x_scale, y_scale, z_scale;
+    int32_t direction; /* +1 forward, -1 backward (advection only) */
+    uint32_t key[2];
+    uint32_t counter[4];
+    uint32_t n_substeps;
+    float min_height_m;
+    float rho_grad_over_rho;
+    uint32_t dry_mode; /* 0 off, 1 scalar v_d, 2 per-slot v_d */
+    float dry_vd;
+    float dry_href;
+    const float *dry_vd_slots;
+    uint32_t wet_mode; /* 0 off, 1 scalar, 2 per-slot, 3 lambda = A * P^B */
+    float wet_lambda, wet_fraction;
+    const float *wet_lambda_slots, *wet_fraction_slots;
+    float wet_a, wet_b;
+    uint32_t loss_mode; /* 0 none, 1 deactivate when mass[0] < loss_min_mass */
+    float loss_min_mass;
+    int32_t n_threads;
+} orc_step_params;
+
+typedef struct {
+    uint32_t nx, ny, nz;
+    uint32_t refine_x, refine_y; /* output cells per met cell (1 = met grid) */
+    uint32_t species;
+    float mass_scale;
+    float outheights[16];
+} orc_grid_params;
+
+void orc_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+void orc_counter_add(const uint32_t counter[4], uint64_t offset, uint32_t out[4]);
+void orc_philox_stream(const uint32_t counter[4], const uint32_t key[2], uint64_t n_blocks,
+                       uint32_t *out);
+float orc_u32_to_uniform01(uint32_t v);
+void orc_box_muller(float u0, float u1, float out[2]);
+
+int orc_step(orc_particle *p, uint64_t n, const orc_met *met, const orc_pbl *pbl,
+             const orc_step_params *sp, uint32_t next_counter[4]);
+
+float orc_update_horizontal(float prev, float sigma, float tl, float dt, float eta);
+float orc_update_vertical(float prev, float sigw, float dsigwdz, float tlw, float dt, float rho,
+                          float eta);
+void orc_deposit(orc_particle *p, uint64_t n, const orc_pbl *pbl, const orc_step_params *sp);
+void orc_advect(orc_particle *p, uint64_t n, const orc_met *met, const orc_step_params *sp);
+
+void orc_hanna(float ust, float wst, float ol, float h, float z, float out[13]);
+
+void orc_grid_accumulate(const orc_particle *p, uint64_t n, const orc_grid_params *gp,
+                         uint32_t *counts, uint32_t *mass_fp);
+
+uint64_t orc_compact(const orc_particle *p, uint64_t n, orc_particle *out,
+                     uint32_t *survivor_map);
+
+uint64_t orc_morton_encode_3d_21bit(uint32_t x, uint32_t y, uint32_t z);
+void orc_morton_keys(const orc_particle *p, uint64_t n, uint32_t bits, int explicit_bounds,
+                     const double bounds[6], uint64_t *keys);
+int orc_sort_morton(const orc_particle *p, uint64_t n, uint32_t bits, int explicit_bounds,
+                    const double bounds[6], orc_particle *out, uint32_t *new_to_old);
+
+void orc_math_eval(int fn, const float *x, const float *y, float *out, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
