@@ -126,6 +126,7 @@ ABI = {
     "fp_compact": (C.c_int, [_vp, C.POINTER(C.c_uint64), _u32]),
     "fp_sort_morton": (C.c_int, [_vp, C.POINTER(SortParams), _u32]),
     "fp_philox_blocks": (C.c_int, [_vp, _u32, _u32, C.c_uint64, _u32]),
+    "fp_math_probe": (C.c_int, [_vp, C.c_int, _f, _f, _f, C.c_uint64]),
     "fp_profile_enable": (C.c_int, [_vp, C.c_int]),
     "fp_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fp_profile_reset": (C.c_int, [_vp]),
@@ -368,6 +369,17 @@ class GpuTimestep:
         self._chk(self._lib.fp_philox_blocks(self._h, c.ctypes.data_as(_u32),
                                              k.ctypes.data_as(_u32), int(n),
                                              out.ctypes.data_as(_u32)))
+        return out
+
+    # -- device arithmetic probe
+    PROBES = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "pow": 4, "sqrt": 5, "div": 6, "log_unit": 7}
+
+    def math_probe(self, fn: str, x, y=None) -> np.ndarray:
+        xs = _f32(x)
+        ys = _f32(np.zeros_like(xs) if y is None else y, xs.size)
+        out = np.zeros_like(xs)
+        self._chk(self._lib.fp_math_probe(self._h, self.PROBES[fn], _fp(xs), _fp(ys), _fp(out),
+                                          xs.size))
         return out
 
     # -- timing / tuning

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "fp_internal.hpp"
+#include "fp_math.h"
 
 static_assert(sizeof(fp_particle) == 96, "fp_particle must match the reference Particle");
 static_assert(sizeof(fp_step_params) == 132, "fp_step_params layout is part of the ABI");
@@ -544,6 +545,7 @@ int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
     a.grid_fused = p->grid_fused ? 1u : 0u;
     if (a.grid_fused) a.grid = grid_args(ctx);
     a.ppt = ctx->ppt;
+    a.log_1em3 = fpm_log_d((double)1.0e-3f);
 
     fp_ctx::Ev ev{};
     prof_begin(ctx, ev, FP_KERNEL_STEP);
@@ -621,6 +623,23 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
     FP_HIP(ctx, dalloc(&d, n * 4));
     hipError_t e = fp::launch_philox_blocks(counter, key, n, d, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    FP_HIP(ctx, e);
+    return FP_OK;
+}
+
+int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n)
+{
+    FP_CHECK(ctx, ctx && x && y && (out || n == 0), "null argument");
+    FP_CHECK(ctx, fn >= FP_PROBE_EXP && fn <= FP_PROBE_LOG_UNIT, "unknown probe function");
+    if (n == 0) return FP_OK;
+    float *d = nullptr;
+    FP_HIP(ctx, dalloc(&d, n * 3));
+    hipError_t e = hipMemcpyAsync(d, x, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + n, y, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = fp::launch_math_probe(fn, d, d + n, d + 2 * n, n, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     FP_HIP(ctx, e);

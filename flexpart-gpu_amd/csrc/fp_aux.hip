@@ -2,6 +2,7 @@
 // AoS <-> SoA transposes, stable stream compaction and the Philox
 // known-answer kernel.
 #include "fp_internal.hpp"
+#include "fp_math.h"
 
 namespace fp {
 namespace {
@@ -176,6 +177,26 @@ __global__ void philox_kernel(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3
     out[4 * i + 0] = x; out[4 * i + 1] = y; out[4 * i + 2] = z; out[4 * i + 3] = w;
 }
 
+__global__ void math_probe_kernel(int fn, const float *x, const float *y, float *out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = x[i], b = y[i];
+    float s, c, r;
+    switch (fn) {
+    case FP_PROBE_EXP: r = fpm_expf(a); break;
+    case FP_PROBE_LOG: r = fpm_logf(a); break;
+    case FP_PROBE_SIN: fpm_sincosf(a, &s, &c); r = s; break;
+    case FP_PROBE_COS: fpm_sincosf(a, &s, &c); r = c; break;
+    case FP_PROBE_POW: r = fpm_powf(a, b); break;
+    case FP_PROBE_SQRT: r = fpm_sqrtf(a); break;
+    case FP_PROBE_DIV: r = a / b; break;
+    case FP_PROBE_LOG_UNIT: r = fpm_logf_unit(a); break;
+    default: r = 0.0f;
+    }
+    out[i] = r;
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -220,6 +241,14 @@ hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *t
     count_kernel<<<ntiles, kBlock, 0, st>>>(in, n, tile_counts);
     scan_kernel<<<1, 1024, 0, st>>>(tile_counts, ntiles, active_count);
     scatter_kernel<<<ntiles, kBlock, 0, st>>>(in, out, n, tile_counts, active_count, survivor_map);
+    return hipGetLastError();
+}
+
+hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
+                             hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    math_probe_kernel<<<blocks_for(n, 256), 256, 0, st>>>(fn, x, y, out, n);
     return hipGetLastError();
 }
 

@@ -84,6 +84,7 @@ struct StepArgs {
     uint32_t grid_fused;
     GridArgs grid;
     uint32_t ppt;           // particles per thread
+    double log_1em3;        // fpm_log_d(1e-3f): the log of the unstable profile's zeta floor
 };
 
 // Which of the two SoA sets holds the current data of each slot range.
@@ -205,6 +206,8 @@ hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *b
 uint64_t compact_blocks(uint32_t n);
 hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2], uint64_t n,
                                 uint32_t *out, hipStream_t st);
+hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
+                             hipStream_t st);
 // Morton sort: keys + stable radix sort + gather (fp_sort.hip).
 hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n);
 

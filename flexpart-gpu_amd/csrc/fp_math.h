@@ -44,27 +44,14 @@ FPM_FN float fpm_u2f(uint32_t u) { float x; __builtin_memcpy(&x, &u, 4); return 
 FPM_FN uint64_t fpm_d2u(double x) { uint64_t u; __builtin_memcpy(&u, &x, 8); return u; }
 FPM_FN double fpm_u2d(uint64_t u) { double x; __builtin_memcpy(&x, &u, 8); return x; }
 
-/* 2^k for a normal exponent k in [-126, 127]. */
-FPM_FN float fpm_pow2i(int k) { return fpm_u2f((uint32_t)(k + 127) << 23); }
-
-/* y * 2^k for y in [0.5, 2) and k in [-152, 130]; one rounding at most. */
-FPM_FN float fpm_scale2(float y, int k)
-{
-    if (k > -126 && k < 128) return y * fpm_pow2i(k);
-    int k1 = k / 2;
-    int k2 = k - k1;
-    return (y * fpm_pow2i(k1)) * fpm_pow2i(k2);
-}
-
-/* e^x, single precision. */
+/* e^x, single precision.  Special cases are selected at the end rather than
+ * branched on, so a wavefront never splits inside the function. */
 FPM_FN float fpm_expf(float x)
 {
-    if (x != x) return x;
-    if (x > 89.0f) return FPM_INF;
-    if (x < -104.0f) return 0.0f;
-    const float k = __builtin_rintf(x * 1.44269502162933349609375f);
+    const float xc = __builtin_fminf(__builtin_fmaxf(x, -104.0f), 89.0f);
+    const float k = __builtin_rintf(xc * 1.44269502162933349609375f);
     /* ln2 = 0.693145751953125 (exact, 12 trailing zero bits) + 1.4286068e-6 */
-    float r = __builtin_fmaf(k, -0.693145751953125f, x);
+    float r = __builtin_fmaf(k, -0.693145751953125f, xc);
     r = __builtin_fmaf(k, -1.428606765330187e-06f, r);
     /* e^r = 1 + r + r^2 * q(r), q from the Taylor series 1/2!, ..., 1/7! */
     float q = 1.98412701e-04f;
@@ -74,30 +61,53 @@ FPM_FN float fpm_expf(float x)
     q = __builtin_fmaf(q, r, 1.66666672e-01f);
     q = __builtin_fmaf(q, r, 0.5f);
     const float p = __builtin_fmaf(q, r * r, r);
-    return fpm_scale2(1.0f + p, (int)k);
+    /* ldexp rounds y * 2^k once, exactly like fpm_scale2; it overflows to
+     * +inf for x in (88.72, 89]. */
+    const float res = __builtin_ldexpf(1.0f + p, (int)k);
+    return x != x ? x : (x < -104.0f ? 0.0f : res);
 }
 
-/* Natural logarithm, single precision. */
+/* Natural logarithm, single precision (special cases selected, not branched). */
 FPM_FN float fpm_logf(float x)
 {
-    if (x != x) return x;
-    if (x < 0.0f) return FPM_NAN;
-    if (x == 0.0f) return -FPM_INF;
-    if (x == FPM_INF) return x;
-    int e = 0;
-    uint32_t ix = fpm_f2u(x);
-    if (ix < 0x00800000u) { /* subnormal: scale into the normal range */
-        ix = fpm_f2u(x * 8388608.0f);
-        e = -23;
-    }
+    const uint32_t ix0 = fpm_f2u(x);
+    const int sub = ix0 < 0x00800000u; /* +0 or subnormal: scale into the normal range */
+    const uint32_t ix = sub ? fpm_f2u(x * 8388608.0f) : ix0;
     /* Split x = 2^e * m with m in [sqrt(1/2), sqrt(2)). */
     const uint32_t t = ix - 0x3f3504f3u;
-    e += ((int32_t)t) >> 23;
+    const int e = (((int32_t)t) >> 23) - (sub ? 23 : 0);
     const float m = fpm_u2f(ix - (t & 0xff800000u));
     const float f = m - 1.0f; /* exact (Sterbenz) */
     const float s = f / (2.0f + f);
     const float z = s * s;
     /* log(1+f) = f - hfsq + s*(hfsq + R), R = 2z/3 + 2z^2/5 + 2z^3/7 + 2z^4/9 */
+    float R = 2.22222224e-01f;
+    R = __builtin_fmaf(R, z, 2.85714298e-01f);
+    R = __builtin_fmaf(R, z, 4.00000006e-01f);
+    R = __builtin_fmaf(R, z, 6.66666687e-01f);
+    R = R * z;
+    const float hfsq = 0.5f * f * f;
+    const float dk = (float)e;
+    float res = dk * 0.693145751953125f -
+                ((hfsq - (s * (hfsq + R) + dk * 1.428606765330187e-06f)) - f);
+    res = x == FPM_INF ? x : res;
+    res = x == 0.0f ? -FPM_INF : res;
+    res = x < 0.0f ? FPM_NAN : res;
+    return x != x ? x : res;
+}
+
+/* fpm_logf restricted to normal x in (0, 1]: the same arithmetic without the
+ * subnormal rescaling and special-value selects, hence the same values on
+ * that domain (the Box-Muller radius argument lies in [2^-126, 1)). */
+FPM_FN float fpm_logf_unit(float x)
+{
+    const uint32_t ix = fpm_f2u(x);
+    const uint32_t t = ix - 0x3f3504f3u;
+    const int e = ((int32_t)t) >> 23;
+    const float m = fpm_u2f(ix - (t & 0xff800000u));
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
     float R = 2.22222224e-01f;
     R = __builtin_fmaf(R, z, 2.85714298e-01f);
     R = __builtin_fmaf(R, z, 4.00000006e-01f);
@@ -133,13 +143,10 @@ FPM_FN void fpm_sincosf(float x, float *sin_out, float *cos_out)
     pc = __builtin_fmaf(pc, z, -0.5f);
     const float cr = __builtin_fmaf(z, pc, 1.0f);
     const int quadrant = ((int)q) & 3;
-    float s, c;
-    if (quadrant == 0) { s = sr; c = cr; }
-    else if (quadrant == 1) { s = cr; c = -sr; }
-    else if (quadrant == 2) { s = -sr; c = -cr; }
-    else { s = -cr; c = sr; }
-    *sin_out = s;
-    *cos_out = c;
+    const float s = (quadrant & 1) ? cr : sr; /* quadrants 1, 3 swap sin and cos */
+    const float c = (quadrant & 1) ? sr : cr;
+    *sin_out = (quadrant & 2) ? -s : s;       /* quadrants 2, 3 negate sin */
+    *cos_out = ((quadrant + 1) & 2) ? -c : c; /* quadrants 1, 2 negate cos */
 }
 
 /* ---- binary64 kernels used by fpm_powf ------------------------------- */
@@ -173,13 +180,12 @@ FPM_FN double fpm_log_d(double x)
 }
 
 /* e^t rounded to single precision, for t in [-104, 89]; ~1e-16 relative
- * before the final rounding. */
+ * before the final rounding (overflow and underflow selected at the end). */
 FPM_FN float fpm_exp_d_to_f(double t)
 {
-    if (t > 89.0) return FPM_INF;
-    if (t < -104.0) return 0.0f;
-    const double k = __builtin_rint(t * 1.4426950408889634);
-    double r = __builtin_fma(k, -0.69314718055989033, t);
+    const double tc = __builtin_fmin(__builtin_fmax(t, -104.0), 89.0);
+    const double k = __builtin_rint(tc * 1.4426950408889634);
+    double r = __builtin_fma(k, -0.69314718055989033, tc);
     r = __builtin_fma(k, -5.497923018708371e-14, r);
     /* Taylor series of e^r to degree 11 */
     double p = 1.0 / 39916800.0;
@@ -196,21 +202,23 @@ FPM_FN float fpm_exp_d_to_f(double t)
     const double y = 1.0 + p;
     const int ki = (int)k;
     /* binary64 scaling is exact for ki in [-152, 130] */
-    const double scaled = y * fpm_u2d((uint64_t)(ki + 1023) << 52);
-    return (float)scaled;
+    float res = (float)(y * fpm_u2d((uint64_t)(ki + 1023) << 52));
+    res = t > 89.0 ? FPM_INF : res;
+    return t < -104.0 ? 0.0f : res;
 }
 
 /* x^y, single precision, for x >= 0 (negative x gives NaN). */
 FPM_FN float fpm_powf(float x, float y)
 {
-    if (y == 0.0f) return 1.0f;
-    if (x != x || y != y) return x + y;
-    if (x == 1.0f) return 1.0f;
-    if (x < 0.0f) return FPM_NAN;
-    if (x == 0.0f) return y > 0.0f ? 0.0f : FPM_INF;
-    if (x == FPM_INF) return y > 0.0f ? FPM_INF : 0.0f;
-    const double t = (double)y * fpm_log_d((double)x);
-    return fpm_exp_d_to_f(t);
+    const float xs = (x > 0.0f && x != FPM_INF) ? x : 1.0f;
+    float res = fpm_exp_d_to_f((double)y * fpm_log_d((double)xs));
+    /* special cases, lowest priority first */
+    res = x == FPM_INF ? (y > 0.0f ? FPM_INF : 0.0f) : res;
+    res = x == 0.0f ? (y > 0.0f ? 0.0f : FPM_INF) : res;
+    res = x < 0.0f ? FPM_NAN : res;
+    res = x == 1.0f ? 1.0f : res;
+    res = (x != x || y != y) ? x + y : res;
+    return y == 0.0f ? 1.0f : res;
 }
 
 FPM_FN float fpm_sqrtf(float x) { return __builtin_sqrtf(x); }

@@ -9,9 +9,17 @@
 // and wet deposition, and written back once (56 B with the velocity memory).
 // Optionally the concentration grid is accumulated in the same pass.
 //
+// The kernel is bound by vector-ALU issue, not by HBM: its arithmetic is laid
+// out so that a wavefront whose particles sit in different turbulence
+// regimes (neutral, unstable, stable Hanna profiles) or Langevin branches
+// still evaluates each expensive primitive - division, exp, the binary64
+// log/exp behind pow, sqrt - once, with regime-selected operands, instead of
+// once per divergent branch.  Every selected operation is the same operation
+// on the same operands as in the reference formula, so values are unchanged.
+//
 // Floating point: compiled with -ffp-contract=off and correctly rounded
 // divide/sqrt; transcendentals come from fp_math.h.  With the operation order
-// kept as in the reference shaders, results are reproducible bit for bit.
+// kept as in the reference, results are reproducible bit for bit.
 #include "fp_internal.hpp"
 #include "fp_math.h"
 
@@ -30,7 +38,9 @@ constexpr float F32_MAX = 3.40282347e38f;
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ bool finite_f(float v) { return (v == v) && (fabsf(v) <= F32_MAX); }
 __device__ __forceinline__ float pos_or_zero(float v) { return (finite_f(v) && v > 0.0f) ? v : 0.0f; }
-__device__ __forceinline__ float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+// a*(1-t) + b*t with the product-sum fused, as the reference CPU twin does
+// (a.mul_add(1.0 - t, b * t), physics/interpolation.rs:76-79); u = 1 - t.
+__device__ __forceinline__ float mix(float a, float b, float t, float u) { return __builtin_fmaf(a, u, b * t); }
 
 // ---------------------------------------------------------------- Philox
 
@@ -40,9 +50,10 @@ __device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        // one 32x32->64 multiply per lane and word (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -60,11 +71,12 @@ __device__ __forceinline__ U4 ctr_add(uint32_t c0, uint32_t c1, uint32_t c2, uin
 
 __device__ __forceinline__ float unit24(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
 
+// box_muller_normals (langevin_fused.wgsl:186-192)
 __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, float &g1)
 {
     const float ru = clampf(unit24(a), 1.1754944e-38f, 0.99999994f);
     const float au = clampf(unit24(b), 0.0f, 0.99999994f);
-    const float rad = fpm_sqrtf(-2.0f * fpm_logf(ru));
+    const float rad = fpm_sqrtf(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1)
     float sn, cs;
     fpm_sincosf(6.2831855f * au, &sn, &cs);
     g0 = rad * cs;
@@ -104,6 +116,9 @@ __device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz,
 
 struct Wind { float u, v, w; };
 
+// sample_dual_u/v/w (advection_dual_wind.wgsl:142-236): trilinear per time
+// level, then the temporal blend.  The 8 corners come from packed 32-byte
+// records {u0,v0,w0,u1,v1,w1,-,-}: two 16-byte loads per corner.
 __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, float x, float y, float z)
 {
     const Axis ax = axis_of(x, a.nx);
@@ -122,14 +137,15 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
         c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
         c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
     }
+    const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - a.alpha;
     float v[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const float x0 = mix(c[0][f], c[1][f], ax.t), x1 = mix(c[2][f], c[3][f], ax.t);
-        const float x2 = mix(c[4][f], c[5][f], ax.t), x3 = mix(c[6][f], c[7][f], ax.t);
-        v[f] = mix(mix(x0, x1, ay.t), mix(x2, x3, ay.t), az.t);
+        const float x0 = mix(c[0][f], c[1][f], ax.t, ux), x1 = mix(c[2][f], c[3][f], ax.t, ux);
+        const float x2 = mix(c[4][f], c[5][f], ax.t, ux), x3 = mix(c[6][f], c[7][f], ax.t, ux);
+        v[f] = mix(mix(x0, x1, ay.t, uy), mix(x2, x3, ay.t, uy), az.t, uz);
     }
-    return Wind{mix(v[0], v[3], a.alpha), mix(v[1], v[4], a.alpha), mix(v[2], v[5], a.alpha)};
+    return Wind{mix(v[0], v[3], a.alpha, ua), mix(v[1], v[4], a.alpha, ua), mix(v[2], v[5], a.alpha, ua)};
 }
 
 __device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
@@ -144,8 +160,11 @@ __device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, f
 
 // ---------------------------------------------------------------- Hanna
 
+// The turbulence regime of a particle is fixed for the whole step: it depends
+// on L and the mixing height of its PBL cell only (langevin_fused.wgsl:236).
 struct Turb {
     float ust, wst, ol, h;
+    bool neu, uns;  // neutral, unstable (else stable)
     float sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
 };
 
@@ -154,86 +173,120 @@ __device__ __forceinline__ bool neutral_regime(float ol, float h)
     return !finite_f(ol) || ol == 0.0f || h / fabsf(ol) < 1.0f;
 }
 
-// Vertical profile (sigma_w, d sigma_w/dz, T_Lw) at height z: the w part of
-// compute_hanna (langevin_fused.wgsl:224-293) and all of hanna_short_w
-// (:296-344), which share their formulas.
-__device__ __forceinline__ void vertical_profile(float ust, float wst, float ol, float h, float z,
-                                                 float zeta, float &sigw, float &dsigwdz, float &tlw)
+// Raw vertical profile at (z, zeta) before the floors: the w part of
+// compute_hanna (langevin_fused.wgsl:224-293) and hanna_short_w (:296-344),
+// which share their formulas.
+//   neutral : corr = z/u*, sigma_w = 1.3 u* e^{-2e-4 corr} (+0.01),
+//             T_Lw = 0.5 z / sigma_w / (1 + 1.5e-3 corr)
+//   unstable: sigma_w^2 = 1.2 w*^2 (1-0.9 zeta) zeta^(2/3) + (1.8-1.4 zeta) u*^2 ...
+//   stable  : sigma_w = 0.01 + 1.3 u* (1-zeta), T_Lw = 0.1 h / sigma_w zeta^0.8
+// Slots shared by all regimes: d0 (one division), E (one exp), d1 and d2 (two
+// divisions by sigma_w and by h or 1+1.5e-3 corr).  zeta^(2/3), zeta^0.8 and
+// max(zeta,1e-3)^(-1/3) are exp(y log zeta) in binary64 and share one log
+// (fpm_powf's own evaluation, so the values are those of fpm_powf).
+struct VRaw { float sigw, sigw_pre_floor, dsigwdz, tlw, d0, d2; };
+
+__device__ __forceinline__ VRaw vertical_raw(const Turb &t, float z, float zeta, double log_1em3)
 {
-    if (neutral_regime(ol, h)) {
-        const float corr = z / ust;
-        sigw = 1.3f * ust * fpm_expf(-2.0e-4f * corr);
-        dsigwdz = -2.0e-4f * sigw;
-        sigw = sigw + SIGMA_FLOOR;
-        tlw = 0.5f * z / sigw / (1.0f + 1.5e-3f * corr);
-    } else if (ol < 0.0f) {
-        const float p23 = fpm_powf(zeta, 2.0f / 3.0f);
-        const float sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * p23 + (1.8f - 1.4f * zeta) * ust * ust;
+    const float ust = t.ust, wst = t.wst, ol = t.ol, h = t.h;
+    const bool neu = t.neu, uns = t.uns;
+    VRaw v;
+    // d0: neutral z/u* (corr), unstable z/L, stable -1.3 u* / h
+    v.d0 = (uns || neu ? z : -1.3f * ust) / (neu ? ust : (uns ? ol : h));
+    // E: neutral e^{-2e-4 corr}, unstable e^{-5 zeta}
+    const float E = fpm_expf(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
+    float pa = 0.0f, pb = 0.0f;
+    if (!neu) {
+        const double L = fpm_log_d((double)(zeta > 0.0f ? zeta : 1.0f));
+        pa = zeta > 0.0f ? fpm_exp_d_to_f((double)(uns ? 2.0f / 3.0f : 0.8f) * L) : 0.0f;
+        if (uns) pb = fpm_exp_d_to_f((double)(-1.0f / 3.0f) * (zeta >= 1.0e-3f ? L : log_1em3));
+    }
+    float sigw, sraw = 0.0f;
+    if (neu) {
+        sraw = 1.3f * ust * E;
+        sigw = sraw + SIGMA_FLOOR;
+    } else if (uns) {
+        const float sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * pa + (1.8f - 1.4f * zeta) * ust * ust;
         sigw = fpm_sqrtf(sq) + SIGMA_FLOOR;
-        dsigwdz = 0.5f / sigw / h *
-                  (-1.4f * ust * ust +
-                   wst * wst * (0.8f * fpm_powf(fmaxf(zeta, 1.0e-3f), -1.0f / 3.0f) - 1.8f * p23));
-        if (z < fabsf(ol)) tlw = 0.1f * z / (sigw * (0.55f - 0.38f * fabsf(z / ol)));
-        else if (zeta < 0.1f) tlw = 0.59f * z / sigw;
-        else tlw = 0.15f * h / sigw * (1.0f - fpm_expf(-5.0f * zeta));
     } else {
         sigw = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
-        dsigwdz = -1.3f * ust / h;
-        tlw = 0.1f * h / sigw * fpm_powf(zeta, 0.8f);
     }
-    tlw = fmaxf(tlw, TLW_MIN);
-    if (dsigwdz == 0.0f) dsigwdz = 1.0e-10f;
-    if (!finite_f(sigw) || sigw <= 0.0f) sigw = SIGMA_FLOOR;
+    v.sigw_pre_floor = sraw;
+    // d1: neutral 0.5 z / sigma_w, unstable 0.5 / sigma_w, stable 0.1 h / sigma_w
+    const float d1 = (neu ? 0.5f * z : (uns ? 0.5f : 0.1f * h)) / sigw;
+    // d2: neutral d1 / (1 + 1.5e-3 corr), unstable d1 / h
+    v.d2 = d1 / (neu ? 1.0f + 1.5e-3f * v.d0 : h);
+    if (neu) {
+        v.dsigwdz = -2.0e-4f * sraw;
+        v.tlw = v.d2;
+    } else if (uns) {
+        v.dsigwdz = v.d2 * (-1.4f * ust * ust + wst * wst * (0.8f * pb - 1.8f * pa));
+        const bool ca = z < fabsf(ol);
+        const bool cb = !ca && zeta < 0.1f;
+        const float d3 = (ca ? 0.1f * z : (cb ? 0.59f * z : 0.15f * h)) /
+                         (ca ? sigw * (0.55f - 0.38f * fabsf(v.d0)) : sigw);
+        v.tlw = (ca || cb) ? d3 : d3 * (1.0f - E);
+    } else {
+        v.dsigwdz = v.d0;
+        v.tlw = d1 * pa;
+    }
+    v.sigw = sigw;
+    return v;
+}
+
+// hanna_short_w: the floors of langevin_fused.wgsl:335-339
+__device__ __forceinline__ void vertical_profile(const Turb &t, float z_in, double log_1em3, float &sigw,
+                                                 float &dsigwdz, float &tlw)
+{
+    const float z = clampf(z_in, 0.0f, t.h);
+    const float zeta = clampf(z / t.h, 0.0f, 1.0f);
+    const VRaw v = vertical_raw(t, z, zeta, log_1em3);
+    tlw = fmaxf(v.tlw, TLW_MIN);
+    dsigwdz = v.dsigwdz == 0.0f ? 1.0e-10f : v.dsigwdz;
+    sigw = (!finite_f(v.sigw) || v.sigw <= 0.0f) ? SIGMA_FLOOR : v.sigw;
     if (!finite_f(tlw)) tlw = TLW_MIN;
     if (!finite_f(dsigwdz)) dsigwdz = 1.0e-10f;
 }
 
-// compute_hanna (langevin_fused.wgsl:224-293): sigma_u,v,w, d sigma_w/dz
-// and Lagrangian time scales at height z.
-__device__ __forceinline__ Turb hanna_full(float ust_in, float wst_in, float ol, float h_in, float z_in)
+// compute_hanna (langevin_fused.wgsl:224-293)
+__device__ __forceinline__ Turb hanna_full(float ust_in, float wst_in, float ol, float h_in, float z_in,
+                                           double log_1em3)
 {
     Turb t;
     t.ust = fmaxf(ust_in, USTAR_MIN);
     t.wst = fmaxf(wst_in, 0.0f);
     t.h = fmaxf(h_in, H_MIN);
     t.ol = ol;
-    const float ust = t.ust, wst = t.wst, h = t.h;
+    t.neu = neutral_regime(ol, t.h);
+    t.uns = !t.neu && ol < 0.0f;
+    const float ust = t.ust, h = t.h;
     const float z = clampf(z_in, 0.0f, h);
     const float zeta = clampf(z / h, 0.0f, 1.0f);
-    float sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
-    if (neutral_regime(ol, h)) {
-        const float corr = z / ust;
-        sigu = SIGMA_FLOOR + 2.0f * ust * fpm_expf(-3.0e-4f * corr);
-        sigw = 1.3f * ust * fpm_expf(-2.0e-4f * corr);
-        dsigwdz = -2.0e-4f * sigw;
-        sigw = sigw + SIGMA_FLOOR;
-        sigv = sigw;
-        tlu = 0.5f * z / sigw / (1.0f + 1.5e-3f * corr);
-        tlv = tlu;
-        tlw = tlu;
-    } else if (ol < 0.0f) {
+    const VRaw v = vertical_raw(t, z, zeta, log_1em3);
+    float sigu, sigv, tlu, tlv;
+    if (t.neu) {
+        sigu = SIGMA_FLOOR + 2.0f * ust * fpm_expf(-3.0e-4f * v.d0);
+        sigv = v.sigw;
+    } else if (t.uns) {
         sigu = SIGMA_FLOOR + ust * fpm_powf(12.0f - 0.5f * h / ol, 1.0f / 3.0f);
         sigv = sigu;
-        const float p23 = fpm_powf(zeta, 2.0f / 3.0f);
-        const float sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * p23 + (1.8f - 1.4f * zeta) * ust * ust;
-        sigw = fpm_sqrtf(sq) + SIGMA_FLOOR;
-        dsigwdz = 0.5f / sigw / h *
-                  (-1.4f * ust * ust +
-                   wst * wst * (0.8f * fpm_powf(fmaxf(zeta, 1.0e-3f), -1.0f / 3.0f) - 1.8f * p23));
-        tlu = 0.15f * h / sigu;
-        tlv = tlu;
-        if (z < fabsf(ol)) tlw = 0.1f * z / (sigw * (0.55f - 0.38f * fabsf(z / ol)));
-        else if (zeta < 0.1f) tlw = 0.59f * z / sigw;
-        else tlw = 0.15f * h / sigw * (1.0f - fpm_expf(-5.0f * zeta));
     } else {
         sigu = SIGMA_FLOOR + 2.0f * ust * (1.0f - zeta);
         sigv = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
-        sigw = sigv;
-        dsigwdz = -1.3f * ust / h;
-        tlu = 0.15f * h / sigu * fpm_sqrtf(zeta);
-        tlv = 0.467f * tlu;
-        tlw = 0.1f * h / sigw * fpm_powf(zeta, 0.8f);
     }
+    // unstable and stable: 0.15 h / sigma_u; neutral: T_Lu = T_Lw
+    const float d5 = (0.15f * h) / sigu;
+    if (t.neu) {
+        tlu = v.d2;
+        tlv = tlu;
+    } else if (t.uns) {
+        tlu = d5;
+        tlv = tlu;
+    } else {
+        tlu = d5 * fpm_sqrtf(zeta);
+        tlv = 0.467f * tlu;
+    }
+    float sigw = v.sigw, dsigwdz = v.dsigwdz, tlw = v.tlw;
     tlu = fmaxf(tlu, TLU_MIN);
     tlv = fmaxf(tlv, TLV_MIN);
     tlw = fmaxf(tlw, TLW_MIN);
@@ -251,21 +304,24 @@ __device__ __forceinline__ Turb hanna_full(float ust_in, float wst_in, float ol,
 }
 
 // ---------------------------------------------------------------- Langevin
+//
+// update_horizontal_component / update_vertical_component
+// (langevin_fused.wgsl:350-389).  Both branches of the reference,
+//   r <  0.5: (1-r) u + eta sigma sqrt(2r)          [+ dt drift]
+//   r >= 0.5: e^{-r} u + eta sigma sqrt(1-e^{-2r})  [+ T_L (1-e^{-r}) drift]
+// are one expression with selected coefficients: one division, one exp and
+// one sqrt per update whichever branch a lane takes.
 
 __device__ __forceinline__ float ou_horizontal(float prev, float sigma_in, float tl_in, float dt, float eta)
 {
     const float sigma = pos_or_zero(sigma_in);
     const float tl = pos_or_zero(tl_in);
-    if (tl == 0.0f) return 0.0f;
     const float r = dt / tl;
-    float nx;
-    if (r < 0.5f) {
-        nx = (1.0f - r) * prev + eta * sigma * fpm_sqrtf(fmaxf(2.0f * r, 0.0f));
-    } else {
-        const float e = fpm_expf(-r);
-        nx = e * prev + eta * sigma * fpm_sqrtf(fmaxf(1.0f - e * e, 0.0f));
-    }
-    return finite_f(nx) ? nx : 0.0f;
+    const bool small = r < 0.5f;
+    const float e = fpm_expf(-r);
+    const float root = fpm_sqrtf(small ? fmaxf(2.0f * r, 0.0f) : fmaxf(1.0f - e * e, 0.0f));
+    const float nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
+    return (tl == 0.0f || !finite_f(nx)) ? 0.0f : nx;
 }
 
 __device__ __forceinline__ float ou_vertical(float prev, float sigw_in, float dsigwdz, float tlw_in,
@@ -273,23 +329,33 @@ __device__ __forceinline__ float ou_vertical(float prev, float sigw_in, float ds
 {
     const float sw = pos_or_zero(sigw_in);
     const float tl = pos_or_zero(tlw_in);
-    if (tl == 0.0f) return 0.0f;
     const float drift = sw * (dsigwdz + rho * sw);
     const float r = dt / tl;
-    float nx;
-    if (r < 0.5f) {
-        nx = (1.0f - r) * prev + eta * sw * fpm_sqrtf(fmaxf(2.0f * r, 0.0f)) + dt * drift;
-    } else {
-        const float e = fpm_expf(-r);
-        nx = e * prev + eta * sw * fpm_sqrtf(fmaxf(1.0f - e * e, 0.0f)) + tl * (1.0f - e) * drift;
+    const bool small = r < 0.5f;
+    const float e = fpm_expf(-r);
+    const float root = fpm_sqrtf(small ? fmaxf(2.0f * r, 0.0f) : fmaxf(1.0f - e * e, 0.0f));
+    const float nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
+                     (small ? dt * drift : tl * (1.0f - e) * drift);
+    return (tl == 0.0f || !finite_f(nx)) ? 0.0f : nx;
+}
+
+// PBL reflection (langevin_fused.wgsl:481-500)
+__device__ __forceinline__ void reflect(float &pz, float &tw, int32_t &cbt, float zmin, float h)
+{
+    if (!(h > zmin)) return;
+    float z = pz;
+    for (int bounce = 0; bounce < 10; ++bounce) {
+        if (z < zmin) { z = 2.0f * zmin - z; cbt = -cbt; tw = -tw; }
+        else if (z > h) { z = 2.0f * h - z; cbt = -cbt; tw = -tw; }
+        else break;
     }
-    return finite_f(nx) ? nx : 0.0f;
+    pz = clampf(z, zmin, h);
 }
 
 // ---------------------------------------------------------------- gridding
 
-__device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, int32_t cx, int32_t cy, float px,
-                                              float py, float z)
+__device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, const float *oh, int32_t cx, int32_t cy,
+                                              float px, float py, float z)
 {
     int32_t fx = (int32_t)floorf(px * (float)g.refine_x);
     int32_t fy = (int32_t)floorf(py * (float)g.refine_y);
@@ -301,10 +367,10 @@ __device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, int32_t cx, int
     iy = iy < 0 ? 0 : (iy > (int64_t)g.ny - 1 ? (int64_t)g.ny - 1 : iy);
     uint32_t iz;
     if (g.metres) {
+        // first level with z <= outheights[k], else the top (concentration_gridding.wgsl:81-93)
         iz = g.nz - 1u;
-        for (uint32_t k = 0; k < g.nz; ++k) {
-            if (z <= g.outheights[k]) { iz = k; break; }
-        }
+        for (uint32_t k = g.nz; k-- > 0;)
+            if (z <= oh[k]) iz = k;
     } else {
         int32_t l = (int32_t)floorf(z);
         iz = (uint32_t)max(0, min(l, (int32_t)g.nz - 1));
@@ -347,11 +413,18 @@ __device__ __forceinline__ void grid_reduce_wave(const GridArgs &g, bool valid, 
     }
 }
 
-template <uint32_t BLOCK>
-__global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
+// NSUB > 0 specialises the vertical sub-step count at compile time (the
+// production value is 4, timeloop.rs:1111); NSUB < 0 reads it at run time.
+// The kernel is latency-bound at low occupancy; the register budget is
+// capped for 3 wavefronts per SIMD (<= 168 VGPRs), the most that fits
+// without spilling to scratch.
+template <uint32_t BLOCK, int NSUB>
+__global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
 {
     __shared__ float lh[kMaxLevels];
+    __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
     for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
+    if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = a.grid.outheights[threadIdx.x];
     __syncthreads();
 
     // XCD-aware chunking: the hardware deals blocks round-robin over the 8
@@ -361,6 +434,7 @@ __global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
     const uint32_t xcd = b % kXcds, k = b / kXcds, per = nb / kXcds, rem = nb % kXcds;
     const uint32_t chunk = xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
     const uint64_t first = (uint64_t)chunk * BLOCK * a.ppt;
+    const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
 
     const Soa s = a.s;
     for (uint32_t r = 0; r < a.ppt; ++r) {
@@ -405,7 +479,7 @@ __global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
             const uint32_t pidx = pi * a.ny + pj;
             const float4 pb = a.pbl[pidx];
             const float ol = fabsf(pb.w) < 1.0e-5f ? __builtin_inff() : 1.0f / pb.w;
-            Turb t = hanna_full(pb.x, pb.y, ol, pb.z, pz);
+            const Turb t = hanna_full(pb.x, pb.y, ol, pb.z, pz, a.log_1em3);
 
             // ---- Philox blocks per slot (langevin_fused.wgsl:430-451)
             const uint64_t cbase = i * a.blocks_per_particle;
@@ -423,31 +497,20 @@ __global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
             tv = ou_horizontal(tv, t.sigv, t.tlv, a.dt, ev);
 
             // ---- vertical OU with sub-steps and reflection
-            if (a.n_substeps == 0u) {
+            if (nsub == 0u) {
                 tw = ou_vertical(tw, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0);
             } else {
-                const float dts = a.dt / (float)a.n_substeps;
+                const float dts = a.dt / (float)nsub;
                 const float zmin = a.min_height;
                 float sigw = t.sigw, dsw = t.dsigwdz, tlw = t.tlw;
-                for (uint32_t sub = 0; sub < a.n_substeps; ++sub) {
+#pragma unroll
+                for (uint32_t sub = 0; sub < (NSUB > 0 ? (uint32_t)NSUB : nsub); ++sub) {
                     const float eta = sub == 0u ? ew0 : (sub == 1u ? ew1 : (sub == 2u ? ew2 : ew3));
                     tw = ou_vertical(tw, sigw, dsw, tlw, dts, a.rho_grad, eta);
                     pz = pz + tw * dts;
-                    if (t.h > zmin) {
-                        float z = pz;
-                        for (int bounce = 0; bounce < 10; ++bounce) {
-                            if (z < zmin) { z = 2.0f * zmin - z; cbt = -cbt; tw = -tw; }
-                            else if (z > t.h) { z = 2.0f * t.h - z; cbt = -cbt; tw = -tw; }
-                            else break;
-                        }
-                        pz = clampf(z, zmin, t.h);
-                    }
-                    if (sub + 1u < a.n_substeps) {
-                        // hanna_short_w at the new height (langevin_fused.wgsl:502-513)
-                        const float zz = clampf(pz, 0.0f, t.h);
-                        const float zeta = clampf(zz / t.h, 0.0f, 1.0f);
-                        vertical_profile(t.ust, t.wst, t.ol, t.h, zz, zeta, sigw, dsw, tlw);
-                    }
+                    reflect(pz, tw, cbt, zmin, t.h);
+                    // hanna_short_w at the new height (langevin_fused.wgsl:502-513)
+                    if (sub + 1u < nsub) vertical_profile(t, pz, a.log_1em3, sigw, dsw, tlw);
                 }
             }
 
@@ -505,7 +568,7 @@ __global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
 
             if (a.grid_fused && (flags & FLAG_ACTIVE)) {
                 grid_valid = true;
-                grid_c = grid_cell(a.grid, cx, cy, px, py, pz);
+                grid_c = grid_cell(a.grid, oh, cx, cy, px, py, pz);
                 grid_q = quantize_mass(mass[a.grid.species], a.grid.mass_scale);
             }
         }
@@ -516,16 +579,27 @@ __global__ void __launch_bounds__(BLOCK) step_kernel(StepArgs a)
 template <uint32_t BLOCK>
 __global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs g)
 {
+    __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
+    if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = g.outheights[threadIdx.x];
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     bool valid = false;
     uint32_t cell = 0, q = 0;
     if (i < n && (s.u(F_FLAGS)[i] & FLAG_ACTIVE)) {
         valid = true;
-        cell = grid_cell(g, s.i(F_CELL_X)[i], s.i(F_CELL_Y)[i], s.f(F_POS_X)[i], s.f(F_POS_Y)[i],
+        cell = grid_cell(g, oh, s.i(F_CELL_X)[i], s.i(F_CELL_Y)[i], s.f(F_POS_X)[i], s.f(F_POS_Y)[i],
                          s.f(F_POS_Z)[i]);
         q = quantize_mass(s.f(F_MASS0 + g.species)[i], g.mass_scale);
     }
     grid_reduce_wave(g, valid, cell, q);
+}
+
+template <uint32_t BLOCK>
+hipError_t launch_block(const StepArgs &a, uint32_t grid, hipStream_t st)
+{
+    if (a.n_substeps == 4u) step_kernel<BLOCK, 4><<<grid, BLOCK, 0, st>>>(a);
+    else step_kernel<BLOCK, -1><<<grid, BLOCK, 0, st>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st)
@@ -534,14 +608,13 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st)
     const uint64_t per_block = (uint64_t)block * a.ppt;
     const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
     switch (block) {
-    case 64: step_kernel<64><<<grid, 64, 0, st>>>(a); break;
-    case 128: step_kernel<128><<<grid, 128, 0, st>>>(a); break;
-    case 256: step_kernel<256><<<grid, 256, 0, st>>>(a); break;
-    case 512: step_kernel<512><<<grid, 512, 0, st>>>(a); break;
-    case 1024: step_kernel<1024><<<grid, 1024, 0, st>>>(a); break;
+    case 64: return launch_block<64>(a, grid, st);
+    case 128: return launch_block<128>(a, grid, st);
+    case 256: return launch_block<256>(a, grid, st);
+    case 512: return launch_block<512>(a, grid, st);
+    case 1024: return launch_block<1024>(a, grid, st);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_grid(const Soa &s, uint32_t n, const GridArgs &g, hipStream_t st)

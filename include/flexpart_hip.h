@@ -209,6 +209,20 @@ int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *params, uint32_t *new_to_o
 int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[2],
                      uint64_t n, uint32_t *out);
 
+/* Evaluates one of the step kernel's arithmetic primitives on the device for
+ * n inputs (host arrays; y is used by pow and div).  Diagnostic: lets a host
+ * check that the device rounds exactly as the CPU does.  No counterpart in
+ * the reference.  fn: FP_PROBE_*.  Blocking. */
+#define FP_PROBE_EXP 0
+#define FP_PROBE_LOG 1
+#define FP_PROBE_SIN 2
+#define FP_PROBE_COS 3
+#define FP_PROBE_POW 4
+#define FP_PROBE_SQRT 5
+#define FP_PROBE_DIV 6
+#define FP_PROBE_LOG_UNIT 7
+int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n);
+
 /* ---- timing ----------------------------------------------------------- */
 
 /* HIP-event timing of each kernel launch on the context stream (the

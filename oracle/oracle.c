@@ -152,7 +152,9 @@ static float max_domain_height(const adv_ctx *a)
     return a->m->level_heights[a->m->nz - 1];
 }
 
-static inline float lerp(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+/* a*(1-t) + b*t with the product-sum fused as the reference's CPU twin does:
+ * a.mul_add(1.0 - t, b * t) (physics/interpolation.rs:76-79). */
+static inline float lerp(float a, float b, float t) { return __builtin_fmaf(a, 1.0f - t, b * t); }
 
 /* clamp_axis advection_dual_wind.wgsl:130-138 */
 static void clamp_axis(float coord, uint32_t n, uint32_t *lo, uint32_t *hi, float *frac)
