@@ -87,9 +87,11 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=144, help="timed steps (144 = the 24 h run)")
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--particles", type=int, default=10_000_000)
-    ap.add_argument("--sort-every", type=int, default=12)
-    ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--ppt", type=int, default=1)
+    ap.add_argument("--sort-every", type=int, default=-1,
+                    help="steps between Morton re-sorts (0 = never, -1 = autotuned)")
+    ap.add_argument("--block", type=int, default=0, help="step-kernel block size (0 = autotuned)")
+    ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
+    ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args(argv)
 
@@ -116,45 +118,56 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
 
     ctx = fp.GpuTimestep(n, w.met.shape, n_species=1, out_shape=w.out_shape, device=0)
-    ctx.set_launch(args.block, args.ppt)
     ctx.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli)
     ctx.configure_grid(w.outheights, w.refine, 0, w.mass_scale)
     steps_per_hour = int(round(3600.0 / w.dt_seconds))
-    sec_per_level = w.met.seconds_between_levels
 
     def reset():
         ctx.upload_store(w.particles)
         ctx.clear_grid()
 
-    def run(n_steps):
+    def step_cfg(k, counter):
+        return fp.StepConfig(dt_seconds=w.dt_seconds, alpha=w.alpha(k * w.dt_seconds)[1],
+                             x_scale=w.x_scale, y_scale=w.y_scale, key=w.key, counter=counter,
+                             dry_mode=1, dry_vd=2e-3, grid_fused=True)
+
+    def run(n_steps, sort_every):
         bracket = -1
         counter = (0, 0, 0, 0)
         for k in range(n_steps):
-            t = k * w.dt_seconds
-            b, alpha = w.alpha(t)
+            b, _ = w.alpha(k * w.dt_seconds)
             if b != bracket:  # upload_dual_wind_if_bracket_changed (timeloop.rs:1378-1402)
                 ptrs = [levels[b][c].data_ptr() for c in range(3)] + \
                        [levels[b + 1][c].data_ptr() for c in range(3)]
                 ctx.upload_met_bracket_dev(ptrs, w.met.level_heights)
                 bracket = b
-            if args.sort_every and k % args.sort_every == 0:
+            if sort_every and k % sort_every == 0:
                 ctx.sort_morton(10, want_map=False)
             if k % steps_per_hour == 0:
                 ctx.clear_grid()
-            cfg = fp.StepConfig(dt_seconds=w.dt_seconds, alpha=alpha, x_scale=w.x_scale,
-                                y_scale=w.y_scale, key=w.key, counter=counter, dry_mode=1,
-                                dry_vd=2e-3, grid_fused=True)
-            counter = ctx.step(cfg)
+            counter = ctx.step(step_cfg(k, counter))
+        return counter
+
+    # autotune launch shape and re-sort interval (untimed) on the state half
+    # way through the run, where the plumes have spread
+    reset()
+    counter = run(args.tune_at, 0)
+    tuned = ctx.autotune(step_cfg(args.tune_at, counter))
+    block = args.block or tuned["block_size"]
+    ppt = args.ppt or tuned["particles_per_thread"]
+    ctx.set_launch(block, ppt)
+    sort_every = tuned["sort_interval"] if args.sort_every < 0 else args.sort_every
+    print(f"# autotune at step {args.tune_at}: {tuned}", file=sys.stderr, flush=True)
 
     reset()
-    run(args.warmup)
+    run(args.warmup, sort_every)
     ctx.sync()
     reset()
     ctx.sync()
     ctx.profile(True)
     ctx.profile_reset()
     t0 = time.perf_counter()
-    run(args.steps)
+    run(args.steps, sort_every)
     ctx.sync()
     elapsed = time.perf_counter() - t0
     step_ms, launches = ctx.profile_read(fp.KERNEL_STEP)
@@ -191,9 +204,10 @@ def main(argv=None) -> int:
             "particles": n,
             "met_grid": list(w.met.shape),
             "dt_seconds": w.dt_seconds,
-            "sort_every": args.sort_every,
-            "block": args.block,
-            "particles_per_thread": args.ppt,
+            "sort_every": sort_every,
+            "block": block,
+            "particles_per_thread": ppt,
+            "autotune": tuned,
             "step_kernel_ms": avg_s * 1e3,
             "sort_ms_per_call": (sort_ms / sorts) if sorts else None,
         },

@@ -97,8 +97,29 @@ class SortParams(C.Structure):
                 ("bounds", C.c_double * 6)]
 
 
+class AutotuneParams(C.Structure):
+    _fields_ = [("warmup_steps", C.c_uint32), ("measured_steps", C.c_uint32),
+                ("sort_bits", C.c_uint32), ("n_sort_intervals", C.c_uint32),
+                ("sort_intervals", C.c_uint32 * 8), ("sort_trial_steps", C.c_uint32),
+                ("cache_path", C.c_char_p)]
+
+
+class AutotuneResult(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("particles_per_thread", C.c_uint32),
+                ("sort_interval", C.c_uint32), ("from_cache", C.c_uint32),
+                ("step_ms", C.c_double), ("step_ms_with_sort", C.c_double)]
+
+
+class EnsDesc(C.Structure):
+    _fields_ = [("n_members", C.c_uint32), ("met_nx", C.c_uint32), ("met_ny", C.c_uint32),
+                ("met_nz", C.c_uint32), ("n_species", C.c_uint32), ("out_nx", C.c_uint32),
+                ("out_ny", C.c_uint32), ("out_nz", C.c_uint32),
+                ("capacities", C.POINTER(C.c_uint64))]
+
+
 _f = C.POINTER(C.c_float)
 _u32 = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
 _vp = C.c_void_p
 
 # name -> (restype, argtypes); every symbol declared in include/flexpart_hip.h
@@ -131,6 +152,24 @@ ABI = {
     "fp_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fp_profile_reset": (C.c_int, [_vp]),
     "fp_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "fp_autotune": (C.c_int, [_vp, C.POINTER(StepParams), C.POINTER(AutotuneParams),
+                              C.POINTER(AutotuneResult)]),
+    "fp_ens_create": (C.c_int, [C.c_int, C.POINTER(EnsDesc), C.POINTER(_vp)]),
+    "fp_ens_destroy": (C.c_int, [_vp]),
+    "fp_ens_last_error": (C.c_char_p, [_vp]),
+    "fp_ens_sync": (C.c_int, [_vp]),
+    "fp_ens_upload_member": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),
+    "fp_ens_download_member": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),
+    "fp_ens_met_upload_member_dev": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
+    "fp_ens_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
+    "fp_ens_grid_configure": (C.c_int, [_vp, C.POINTER(GridParams)]),
+    "fp_ens_grid_clear": (C.c_int, [_vp]),
+    "fp_ens_grid_download_member": (C.c_int, [_vp, C.c_uint32, _u32, _u32]),
+    "fp_ens_configure": (C.c_int, [_vp, C.POINTER(StepParams), _u32, _u32, _u64, C.c_double,
+                                   C.c_double]),
+    "fp_ens_step": (C.c_int, [_vp, C.c_uint32, C.c_int, _u32]),
+    "fp_ens_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "fp_ens_last_step_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
 }
 
 _LIB: C.CDLL | None = None
@@ -316,13 +355,7 @@ class GpuTimestep:
 
     # -- concentration grid
     def configure_grid(self, outheights=None, refine=(1, 1), species=0, mass_scale=1.0e6):
-        g = GridParams()
-        g.refine_x, g.refine_y = refine
-        g.species = species
-        g.mass_scale = mass_scale
-        if outheights is not None:
-            for i, v in enumerate(outheights):
-                g.outheights[i] = float(v)
+        g = _grid_params(outheights, refine, species, mass_scale)
         self._chk(self._lib.fp_grid_configure(self._h, C.byref(g)))
 
     def clear_grid(self):
@@ -398,8 +431,156 @@ class GpuTimestep:
     def set_launch(self, block_size: int = 256, particles_per_thread: int = 1):
         self._chk(self._lib.fp_set_launch(self._h, block_size, particles_per_thread))
 
+    def autotune(self, cfg: StepConfig, sort_intervals=(0, 6, 12, 36), warmup_steps: int = 2,
+                 measured_steps: int = 7, sort_trial_steps: int = 36, sort_bits: int = 10,
+                 cache_path: str | None = None) -> dict:
+        """Pick block size, particles/thread and re-sort interval (fp_autotune)."""
+        p = AutotuneParams()
+        p.warmup_steps, p.measured_steps = warmup_steps, measured_steps
+        p.sort_bits, p.sort_trial_steps = sort_bits, sort_trial_steps
+        p.n_sort_intervals = len(sort_intervals)
+        for i, s in enumerate(sort_intervals):
+            p.sort_intervals[i] = s
+        p.cache_path = cache_path.encode() if cache_path else None
+        r = AutotuneResult()
+        self._chk(self._lib.fp_autotune(self._h, C.byref(cfg.to_c()), C.byref(p), C.byref(r)))
+        return {"block_size": r.block_size, "particles_per_thread": r.particles_per_thread,
+                "sort_interval": r.sort_interval, "from_cache": bool(r.from_cache),
+                "step_ms": r.step_ms, "step_ms_with_sort": r.step_ms_with_sort}
+
+
+def _grid_params(outheights, refine, species, mass_scale) -> GridParams:
+    g = GridParams()
+    g.refine_x, g.refine_y = refine
+    g.species = species
+    g.mass_scale = mass_scale
+    if outheights is not None:
+        for i, v in enumerate(outheights):
+            g.outheights[i] = float(v)
+    return g
+
+
+class GpuEnsemble:
+    """Many independent members advanced by one launch per step (fp_ens_*).
+
+    Members share the vertical grid, the PBL state and the step parameters;
+    each has its own particles, met bracket, Philox key and counter and
+    output grid.  Every member's result equals what ``GpuTimestep.step``
+    gives for that member alone with the same key, counter and alpha.
+    """
+
+    def __init__(self, capacities, met_shape, n_species: int = 4, out_shape=None,
+                 device: int = 0):
+        lib = library()
+        caps = np.ascontiguousarray(np.asarray(capacities, np.uint64))
+        nx, ny, nz = met_shape
+        ox, oy, oz = out_shape if out_shape is not None else (0, 0, 0)
+        d = EnsDesc(len(caps), nx, ny, nz, n_species, ox, oy, oz, caps.ctypes.data_as(_u64))
+        h = _vp()
+        rc = lib.fp_ens_create(device, C.byref(d), C.byref(h))
+        if rc != FP_OK:
+            raise FpError(rc, lib.fp_ens_last_error(None).decode())
+        self._lib = lib
+        self._h = h
+        self.capacities = caps.copy()
+        self.n_members = len(caps)
+        self.met_shape = (nx, ny, nz)
+        self.out_shape = (ox, oy, oz)
+        self.n_species = n_species
+
+    def _chk(self, rc):
+        if rc != FP_OK:
+            raise FpError(rc, self._lib.fp_ens_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.fp_ens_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._chk(self._lib.fp_ens_sync(self._h))
+
+    def upload_member(self, member: int, particles: np.ndarray):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        self._chk(self._lib.fp_ens_upload_member(self._h, member, p.ctypes.data, len(p)))
+
+    def download_member(self, member: int, n=None) -> np.ndarray:
+        n = int(self.capacities[member]) if n is None else int(n)
+        out = np.zeros(n, PARTICLE_DTYPE)
+        self._chk(self._lib.fp_ens_download_member(self._h, member, out.ctypes.data, n))
+        return out
+
+    def upload_met_member_dev(self, member: int, ptrs, level_heights):
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_ens_met_upload_member_dev(self._h, member,
+                                                         *[_vp(int(p)) for p in ptrs], _fp(lh)))
+
+    def upload_pbl(self, ustar, wstar, hmix, oli, precip=None):
+        n = self.met_shape[0] * self.met_shape[1]
+        arrs = [_f32(a, n) for a in (ustar, wstar, hmix, oli)]
+        pr = None if precip is None else _f32(precip, n)
+        self._chk(self._lib.fp_ens_pbl_upload(self._h, *[_fp(a) for a in arrs], _fp(pr)))
+
+    def configure_grid(self, outheights=None, refine=(1, 1), species=0, mass_scale=1.0e6):
+        g = _grid_params(outheights, refine, species, mass_scale)
+        self._chk(self._lib.fp_ens_grid_configure(self._h, C.byref(g)))
+
+    def clear_grid(self):
+        self._chk(self._lib.fp_ens_grid_clear(self._h))
+
+    def download_grid_member(self, member: int):
+        n = int(np.prod(self.out_shape))
+        counts = np.zeros(n, np.uint32)
+        mass = np.zeros(n, np.uint32)
+        self._chk(self._lib.fp_ens_grid_download_member(self._h, member, counts.ctypes.data_as(_u32),
+                                                        mass.ctypes.data_as(_u32)))
+        return counts, mass
+
+    def configure(self, cfg: StepConfig, keys=None, counters=None, dispatch=None,
+                  time_in_bracket_s: float = 0.0, bracket_s: float = 10800.0):
+        """Step parameters, per-member keys (M x 2) / counters (M x 4) / dispatch."""
+        k = None if keys is None else np.ascontiguousarray(np.asarray(keys, np.uint32).reshape(-1))
+        c = None if counters is None else np.ascontiguousarray(np.asarray(counters, np.uint32).reshape(-1))
+        d = None if dispatch is None else np.ascontiguousarray(np.asarray(dispatch, np.uint64))
+        for arr, per in ((k, 2), (c, 4), (d, 1)):
+            if arr is not None and arr.size != per * self.n_members:
+                raise ValueError(f"expected {per} values per member")
+        self._chk(self._lib.fp_ens_configure(
+            self._h, C.byref(cfg.to_c()),
+            None if k is None else k.ctypes.data_as(_u32),
+            None if c is None else c.ctypes.data_as(_u32),
+            None if d is None else d.ctypes.data_as(_u64),
+            float(time_in_bracket_s), float(bracket_s)))
+
+    def step(self, n_steps: int = 1, use_graph: bool = False) -> np.ndarray:
+        """n_steps of every member; returns the (M, 4) counters after them."""
+        out = np.zeros((self.n_members, 4), np.uint32)
+        self._chk(self._lib.fp_ens_step(self._h, int(n_steps), 1 if use_graph else 0,
+                                        out.ctypes.data_as(_u32)))
+        return out
+
+    def set_launch(self, block_size: int = 256, particles_per_thread: int = 1):
+        self._chk(self._lib.fp_ens_set_launch(self._h, block_size, particles_per_thread))
+
+    def last_step_ms(self) -> float:
+        ms = C.c_double()
+        self._chk(self._lib.fp_ens_last_step_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
 
 __all__ = [
-    "GpuTimestep", "StepConfig", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
+    "GpuTimestep", "GpuEnsemble", "StepConfig", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
     "library", "ABI", "LIB_PATH",
 ]

@@ -15,6 +15,8 @@ static_assert(sizeof(fp_step_params) == 132, "fp_step_params layout is part of t
 static_assert(sizeof(fp_ctx_desc) == 64, "fp_ctx_desc layout is part of the ABI");
 static_assert(sizeof(fp_grid_params) == 80, "fp_grid_params layout is part of the ABI");
 static_assert(sizeof(fp_sort_params) == 56, "fp_sort_params layout is part of the ABI");
+static_assert(sizeof(fp_autotune_params) == 64, "fp_autotune_params layout is part of the ABI");
+static_assert(sizeof(fp_autotune_result) == 32, "fp_autotune_result layout is part of the ABI");
 
 namespace {
 
@@ -148,6 +150,70 @@ void destroy(fp_ctx *ctx)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- shared helpers
+
+fp::GridArgs fp::make_grid_args(uint32_t *counts, uint32_t *mass_fp, uint32_t nx, uint32_t ny, uint32_t nz,
+                                const fp_grid_params &gp)
+{
+    fp::GridArgs g{};
+    g.counts = counts;
+    g.mass_fp = mass_fp;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.refine_x = gp.refine_x;
+    g.refine_y = gp.refine_y;
+    g.species = gp.species;
+    g.mass_scale = gp.mass_scale;
+    g.metres = nz ? (gp.outheights[nz - 1] > 0.0f) : 0u;
+    for (int k = 0; k < FP_MAX_OUTPUT_LEVELS; ++k) g.outheights[k] = gp.outheights[k];
+    return g;
+}
+
+void fp::fill_step_scalars(fp::StepArgs &a, const fp_step_params &p)
+{
+    a.dt = p.dt_seconds;
+    a.dt_adv = p.direction < 0 ? -p.dt_seconds : p.dt_seconds;
+    a.alpha = std::min(std::max(p.alpha, 0.0f), 1.0f);  // gpu/advection.rs:639
+    a.x_scale = p.x_scale;
+    a.y_scale = p.y_scale;
+    a.z_scale = p.z_scale;
+    a.key0 = p.key[0];
+    a.key1 = p.key[1];
+    a.c0 = p.counter[0];
+    a.c1 = p.counter[1];
+    a.c2 = p.counter[2];
+    a.c3 = p.counter[3];
+    a.n_substeps = p.n_substeps;
+    const uint32_t nv = std::max(p.n_substeps, 1u);
+    a.blocks_per_particle = (2u + nv + 3u) / 4u;  // langevin_fused.wgsl:435-436
+    a.min_height = p.min_height_m;
+    a.rho_grad = p.rho_grad_over_rho;
+    a.dry_mode = p.dry_mode;
+    a.dry_vd = p.dry_vd;
+    a.dry_href = p.dry_href;
+    a.wet_mode = p.wet_mode;
+    a.wet_lambda = p.wet_lambda;
+    a.wet_fraction = p.wet_fraction;
+    a.wet_a = p.wet_a;
+    a.wet_b = p.wet_b;
+    a.loss_mode = p.loss_mode;
+    a.loss_min_mass = p.loss_min_mass;
+    a.grid_fused = p.grid_fused ? 1u : 0u;
+    a.log_1em3 = fpm_log_d((double)1.0e-3f);
+}
+
+void fp::counter_advance(const uint32_t c[4], uint64_t off, uint32_t out[4])
+{
+    const uint64_t lo = (uint64_t)c[0] + (uint32_t)off;
+    const uint64_t mid = (uint64_t)c[1] + (uint32_t)(off >> 32) + (lo >> 32);
+    const uint64_t hi = (uint64_t)c[2] + (mid >> 32);
+    out[0] = (uint32_t)lo;
+    out[1] = (uint32_t)mid;
+    out[2] = (uint32_t)hi;
+    out[3] = c[3] + (uint32_t)(hi >> 32);
+}
 
 extern "C" {
 
@@ -419,19 +485,8 @@ int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,
 
 static fp::GridArgs grid_args(const fp_ctx *ctx)
 {
-    fp::GridArgs g{};
-    g.counts = ctx->grid_counts;
-    g.mass_fp = ctx->grid_mass;
-    g.nx = ctx->desc.out_nx;
-    g.ny = ctx->desc.out_ny;
-    g.nz = ctx->desc.out_nz;
-    g.refine_x = ctx->grid_params.refine_x;
-    g.refine_y = ctx->grid_params.refine_y;
-    g.species = ctx->grid_params.species;
-    g.mass_scale = ctx->grid_params.mass_scale;
-    g.metres = g.nz ? (ctx->grid_params.outheights[g.nz - 1] > 0.0f) : 0u;
-    for (int k = 0; k < FP_MAX_OUTPUT_LEVELS; ++k) g.outheights[k] = ctx->grid_params.outheights[k];
-    return g;
+    return fp::make_grid_args(ctx->grid_counts, ctx->grid_mass, ctx->desc.out_nx, ctx->desc.out_ny,
+                              ctx->desc.out_nz, ctx->grid_params);
 }
 
 int fp_grid_configure(fp_ctx *ctx, const fp_grid_params *params)
@@ -515,56 +570,20 @@ int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
     a.dry_vd_slots = ctx->dry_vd;
     a.wet_lambda_slots = ctx->wet_lambda;
     a.wet_fraction_slots = ctx->wet_fraction;
-    a.dt = p->dt_seconds;
-    a.dt_adv = p->direction < 0 ? -p->dt_seconds : p->dt_seconds;
-    a.alpha = std::min(std::max(p->alpha, 0.0f), 1.0f);  // gpu/advection.rs:639
-    a.x_scale = p->x_scale;
-    a.y_scale = p->y_scale;
-    a.z_scale = p->z_scale;
-    a.key0 = p->key[0];
-    a.key1 = p->key[1];
-    a.c0 = p->counter[0];
-    a.c1 = p->counter[1];
-    a.c2 = p->counter[2];
-    a.c3 = p->counter[3];
-    a.n_substeps = p->n_substeps;
-    const uint32_t nv = std::max(p->n_substeps, 1u);
-    a.blocks_per_particle = (2u + nv + 3u) / 4u;  // langevin_fused.wgsl:435-436
-    a.min_height = p->min_height_m;
-    a.rho_grad = p->rho_grad_over_rho;
-    a.dry_mode = p->dry_mode;
-    a.dry_vd = p->dry_vd;
-    a.dry_href = p->dry_href;
-    a.wet_mode = p->wet_mode;
-    a.wet_lambda = p->wet_lambda;
-    a.wet_fraction = p->wet_fraction;
-    a.wet_a = p->wet_a;
-    a.wet_b = p->wet_b;
-    a.loss_mode = p->loss_mode;
-    a.loss_min_mass = p->loss_min_mass;
-    a.grid_fused = p->grid_fused ? 1u : 0u;
+    fp::fill_step_scalars(a, *p);
     if (a.grid_fused) a.grid = grid_args(ctx);
     a.ppt = ctx->ppt;
-    a.log_1em3 = fpm_log_d((double)1.0e-3f);
 
     fp_ctx::Ev ev{};
     prof_begin(ctx, ev, FP_KERNEL_STEP);
     FP_HIP(ctx, fp::launch_step(a, ctx->block_size, ctx->stream));
     prof_end(ctx, ev);
 
-    if (next_counter) {
-        // host counter advance (gpu/langevin_fused.rs:229-233), 128-bit add
-        const uint64_t off = ctx->dispatch * (p->n_substeps <= 2 ? 1ull : 2ull);
-        const uint64_t lo = (uint64_t)p->counter[0] + (uint32_t)off;
-        const uint64_t mid = (uint64_t)p->counter[1] + (uint32_t)(off >> 32) + (lo >> 32);
-        const uint64_t hi = (uint64_t)p->counter[2] + (mid >> 32);
-        next_counter[0] = (uint32_t)lo;
-        next_counter[1] = (uint32_t)mid;
-        next_counter[2] = (uint32_t)hi;
-        next_counter[3] = p->counter[3] + (uint32_t)(hi >> 32);
-    }
+    if (next_counter)
+        fp::counter_advance(p->counter, fp::counter_per_step(ctx->dispatch, p->n_substeps), next_counter);
     return FP_OK;
 }
+
 
 // ---------------------------------------------------------------- compaction / sort
 
@@ -597,6 +616,7 @@ int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *p, uint32_t *new_to_old)
     FP_CHECK(ctx, p->bits_per_axis >= 1 && p->bits_per_axis <= 21, "bits_per_axis must be 1..21");
     const uint32_t n = (uint32_t)ctx->dispatch;
     if (n == 0) return FP_OK;
+    if (int rc = ensure_stage(ctx, n)) return rc;
     fp_ctx::Ev ev{};
     prof_begin(ctx, ev, FP_KERNEL_SORT);
     FP_HIP(ctx, fp::sort_morton(ctx, *p, n));

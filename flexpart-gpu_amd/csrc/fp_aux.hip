@@ -40,13 +40,37 @@ __global__ void aos_to_soa_kernel(const uint32_t *aos, const uint32_t *slots, ui
     for (uint32_t f = 0; f < F_COUNT; ++f) s.u(f)[slot] = src[f];
 }
 
-__global__ void soa_to_aos_kernel(Soa s, uint32_t *aos, uint64_t n)
+// Tiled transposes through LDS for whole ranges: a block moves kT records;
+// SoA side reads/writes each field coalesced, AoS side reads/writes the
+// block's contiguous kT*96-byte span coalesced.
+constexpr uint32_t kT = 256;
+
+__global__ void __launch_bounds__(kT) soa_to_aos_tiled(Soa s, uint32_t *aos, uint64_t n)
 {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    uint32_t *dst = aos + r * F_COUNT;
+    __shared__ uint32_t tile[F_COUNT][kT + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kT;
+    const uint32_t t = threadIdx.x;
+    const uint32_t cnt = (uint32_t)min<uint64_t>(kT, n - base);
+    if (t < cnt)
 #pragma unroll
-    for (uint32_t f = 0; f < F_COUNT; ++f) dst[f] = s.u(f)[r];
+        for (uint32_t f = 0; f < F_COUNT; ++f) tile[f][t] = s.u(f)[base + t];
+    __syncthreads();
+    uint32_t *dst = aos + base * F_COUNT;
+    for (uint32_t w = t; w < cnt * F_COUNT; w += kT) dst[w] = tile[w % F_COUNT][w / F_COUNT];
+}
+
+__global__ void __launch_bounds__(kT) aos_to_soa_tiled(const uint32_t *aos, uint64_t n, Soa s)
+{
+    __shared__ uint32_t tile[F_COUNT][kT + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kT;
+    const uint32_t t = threadIdx.x;
+    const uint32_t cnt = (uint32_t)min<uint64_t>(kT, n - base);
+    const uint32_t *src = aos + base * F_COUNT;
+    for (uint32_t w = t; w < cnt * F_COUNT; w += kT) tile[w % F_COUNT][w / F_COUNT] = src[w];
+    __syncthreads();
+    if (t < cnt)
+#pragma unroll
+        for (uint32_t f = 0; f < F_COUNT; ++f) s.u(f)[base + t] = tile[f][t];
 }
 
 // ---------------------------------------------------------------- compaction
@@ -219,15 +243,18 @@ hipError_t launch_aos_to_soa(const fp_particle *aos, const uint32_t *slots, uint
                              hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    aos_to_soa_kernel<<<blocks_for(n, 256), 256, 0, st>>>(
-        reinterpret_cast<const uint32_t *>(aos), slots, n, s);
+    if (slots)
+        aos_to_soa_kernel<<<blocks_for(n, 256), 256, 0, st>>>(
+            reinterpret_cast<const uint32_t *>(aos), slots, n, s);
+    else
+        aos_to_soa_tiled<<<blocks_for(n, kT), kT, 0, st>>>(reinterpret_cast<const uint32_t *>(aos), n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_soa_to_aos(Soa s, fp_particle *aos, uint64_t n, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    soa_to_aos_kernel<<<blocks_for(n, 256), 256, 0, st>>>(s, reinterpret_cast<uint32_t *>(aos), n);
+    soa_to_aos_tiled<<<blocks_for(n, kT), kT, 0, st>>>(s, reinterpret_cast<uint32_t *>(aos), n);
     return hipGetLastError();
 }
 

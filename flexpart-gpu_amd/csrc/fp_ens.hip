@@ -1,0 +1,449 @@
+// fp_ens.hip - batched ensembles: many independent members, one launch per
+// sync step, optionally replayed from a captured hipGraph (fp_ens_* in
+// include/flexpart_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fp_internal.hpp"
+#include "fp_math.h"
+
+struct fp_ens {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint32_t n_members = 0;
+    uint32_t nx = 0, ny = 0, nz = 0, n_species = 1;
+    uint32_t ox = 0, oy = 0, oz = 0;
+    std::vector<uint64_t> cap, off;
+    uint64_t total = 0, max_cap = 0;
+    fp::Soa soa;
+
+    float4 *met = nullptr;  // n_members * npts * 2
+    float *level_heights = nullptr;
+    std::vector<float> lh_host;
+    bool has_heights = false;
+    float zmax = 0.0f;
+    std::vector<char> met_ready;
+
+    float4 *pbl = nullptr;
+    float *precip = nullptr;
+    float *pbl_stage = nullptr;
+    bool pbl_ready = false, precip_ready = false;
+
+    uint32_t *grid_counts = nullptr, *grid_mass = nullptr;
+    fp_grid_params gp{};
+    bool grid_configured = false;
+
+    fp_particle *aos_stage = nullptr;
+
+    fp::EnsMember *d_members = nullptr;
+    fp::EnsBlock *d_blocks = nullptr;
+    uint64_t d_blocks_cap = 0;
+    fp::EnsClock *d_clock = nullptr;
+    std::vector<fp::EnsMember> members;
+    uint32_t nblocks = 0;
+    bool configured = false, sub4 = false;
+    uint64_t steps_done = 0;
+
+    uint32_t block = 256, ppt = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    double last_ms = 0.0;
+
+    uint64_t npts() const { return (uint64_t)nx * ny * nz; }
+    uint64_t ncell() const { return (uint64_t)ox * oy * oz; }
+};
+
+namespace {
+
+thread_local std::string g_ens_error;
+
+int efail(fp_ens *e, int code, const std::string &msg)
+{
+    if (e) e->err = msg; else g_ens_error = msg;
+    return code;
+}
+
+#define E_HIP(e, expr)                                                                        \
+    do {                                                                                      \
+        hipError_t h_ = (expr);                                                               \
+        if (h_ != hipSuccess)                                                                 \
+            return efail((e), h_ == hipErrorOutOfMemory ? FP_E_NOMEM : FP_E_DEVICE,           \
+                         std::string(#expr) + ": " + hipGetErrorString(h_));                  \
+    } while (0)
+#define E_CHECK(e, cond, msg)                                                                 \
+    do {                                                                                      \
+        if (!(cond)) return efail((e), FP_E_INVALID, (msg));                                  \
+    } while (0)
+#define E_STATE(e, cond, msg)                                                                 \
+    do {                                                                                      \
+        if (!(cond)) return efail((e), FP_E_STATE, (msg));                                    \
+    } while (0)
+
+template <typename T>
+hipError_t dmalloc(T **p, uint64_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+void drop_graph(fp_ens *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+void destroy(fp_ens *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    fr(e->soa.base); fr(e->met); fr(e->level_heights); fr(e->pbl); fr(e->precip); fr(e->pbl_stage);
+    fr(e->grid_counts); fr(e->grid_mass); fr(e->aos_stage); fr(e->d_members); fr(e->d_blocks);
+    fr(e->d_clock);
+    if (e->ev_a) (void)hipEventDestroy(e->ev_a);
+    if (e->ev_b) (void)hipEventDestroy(e->ev_b);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fp_ens_last_error(const fp_ens *e) { return e ? e->err.c_str() : g_ens_error.c_str(); }
+
+int fp_ens_create(int device_ordinal, const fp_ens_desc *desc, fp_ens **out)
+{
+    if (!desc || !out || !desc->capacities) return efail(nullptr, FP_E_INVALID, "null argument");
+    *out = nullptr;
+    const fp_ens_desc &d = *desc;
+    if (d.n_members == 0) return efail(nullptr, FP_E_INVALID, "n_members must be >= 1");
+    if (d.met_nx == 0 || d.met_ny == 0 || d.met_nz == 0 || d.met_nz > fp::kMaxLevels)
+        return efail(nullptr, FP_E_INVALID, "met grid dimensions must be in [1, 256] levels");
+    if (d.n_species < 1 || d.n_species > FP_MAX_SPECIES) return efail(nullptr, FP_E_INVALID, "n_species must be 1..4");
+    if (d.out_nz > FP_MAX_OUTPUT_LEVELS) return efail(nullptr, FP_E_INVALID, "out_nz exceeds 16");
+    uint64_t total = 0, mx = 0;
+    for (uint32_t m = 0; m < d.n_members; ++m) {
+        total += d.capacities[m];
+        mx = std::max<uint64_t>(mx, d.capacities[m]);
+    }
+    if (total == 0 || total > 0x7fffffffull) return efail(nullptr, FP_E_INVALID, "total capacity must be in [1, 2^31)");
+    const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
+    if (npts * 2 >= 0x80000000ull) return efail(nullptr, FP_E_INVALID, "met grid too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return efail(nullptr, FP_E_DEVICE, "no HIP device");
+    if (device_ordinal < 0 || device_ordinal >= ndev) return efail(nullptr, FP_E_INVALID, "device ordinal out of range");
+    if (hipSetDevice(device_ordinal) != hipSuccess) return efail(nullptr, FP_E_DEVICE, "hipSetDevice failed");
+
+    fp_ens *e = new (std::nothrow) fp_ens();
+    if (!e) return efail(nullptr, FP_E_NOMEM, "host allocation failed");
+    e->device = device_ordinal;
+    e->n_members = d.n_members;
+    e->nx = d.met_nx; e->ny = d.met_ny; e->nz = d.met_nz;
+    e->n_species = d.n_species;
+    e->ox = d.out_nx; e->oy = d.out_ny; e->oz = d.out_nz;
+    e->cap.assign(d.capacities, d.capacities + d.n_members);
+    e->off.resize(d.n_members);
+    uint64_t acc = 0;
+    for (uint32_t m = 0; m < d.n_members; ++m) { e->off[m] = acc; acc += e->cap[m]; }
+    e->total = total;
+    e->max_cap = mx;
+    e->met_ready.assign(d.n_members, 0);
+    auto bail = [&](hipError_t h, const char *what) {
+        g_ens_error = std::string(what) + ": " + hipGetErrorString(h);
+        destroy(e);
+        return h == hipErrorOutOfMemory ? FP_E_NOMEM : FP_E_DEVICE;
+    };
+#define C_HIP(expr)                                  \
+    do {                                             \
+        hipError_t h_ = (expr);                      \
+        if (h_ != hipSuccess) return bail(h_, #expr); \
+    } while (0)
+    C_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->soa.capacity = total;
+    C_HIP(dmalloc(&e->soa.base, total * fp::F_COUNT));
+    C_HIP(hipMemsetAsync(e->soa.base, 0, total * fp::F_COUNT * 4, e->stream));
+    C_HIP(dmalloc(&e->met, npts * 2 * d.n_members));
+    C_HIP(dmalloc(&e->level_heights, (uint64_t)d.met_nz));
+    const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
+    C_HIP(dmalloc(&e->pbl, np));
+    C_HIP(dmalloc(&e->precip, np));
+    C_HIP(dmalloc(&e->pbl_stage, np * 4));
+    if (e->ncell()) {
+        C_HIP(dmalloc(&e->grid_counts, e->ncell() * d.n_members));
+        C_HIP(dmalloc(&e->grid_mass, e->ncell() * d.n_members));
+        C_HIP(hipMemsetAsync(e->grid_counts, 0, e->ncell() * d.n_members * 4, e->stream));
+        C_HIP(hipMemsetAsync(e->grid_mass, 0, e->ncell() * d.n_members * 4, e->stream));
+    }
+    C_HIP(dmalloc(&e->aos_stage, std::max<uint64_t>(mx, 1)));
+    C_HIP(dmalloc(&e->d_members, (uint64_t)d.n_members));
+    C_HIP(dmalloc(&e->d_clock, 1));
+    C_HIP(hipEventCreate(&e->ev_a));
+    C_HIP(hipEventCreate(&e->ev_b));
+    C_HIP(hipStreamSynchronize(e->stream));
+#undef C_HIP
+    e->gp.refine_x = e->gp.refine_y = 1;
+    e->gp.mass_scale = 1.0e6f;
+    *out = e;
+    return FP_OK;
+}
+
+int fp_ens_destroy(fp_ens *e)
+{
+    destroy(e);
+    return FP_OK;
+}
+
+int fp_ens_sync(fp_ens *e)
+{
+    E_CHECK(e, e, "null ensemble");
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+int fp_ens_upload_member(fp_ens *e, uint32_t m, const fp_particle *p, uint64_t n)
+{
+    E_CHECK(e, e && (p || n == 0), "null argument");
+    E_CHECK(e, m < e->n_members && n <= e->cap[m], "member index or count out of range");
+    for (uint64_t r = 0; r < n; ++r)
+        for (uint32_t s = e->n_species; s < FP_MAX_SPECIES; ++s)
+            E_CHECK(e, p[r].mass[s] == 0.0f, "mass slot beyond n_species is non-zero");
+    if (n == 0) return FP_OK;
+    fp::Soa s = e->soa;
+    s.base += e->off[m];
+    E_HIP(e, hipMemcpyAsync(e->aos_stage, p, n * sizeof(fp_particle), hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, fp::launch_aos_to_soa(e->aos_stage, nullptr, n, s, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+int fp_ens_download_member(fp_ens *e, uint32_t m, fp_particle *out, uint64_t n)
+{
+    E_CHECK(e, e && (out || n == 0), "null argument");
+    E_CHECK(e, m < e->n_members && n <= e->cap[m], "member index or count out of range");
+    if (n == 0) return FP_OK;
+    fp::Soa s = e->soa;
+    s.base += e->off[m];
+    E_HIP(e, fp::launch_soa_to_aos(s, e->aos_stage, n, e->stream));
+    E_HIP(e, hipMemcpyAsync(out, e->aos_stage, n * sizeof(fp_particle), hipMemcpyDeviceToHost, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+int fp_ens_met_upload_member_dev(fp_ens *e, uint32_t m, const float *u0, const float *v0, const float *w0,
+                                 const float *u1, const float *v1, const float *w1, const float *lh)
+{
+    E_CHECK(e, e && u0 && v0 && w0 && u1 && v1 && w1 && lh, "null argument");
+    E_CHECK(e, m < e->n_members, "member index out of range");
+    for (uint32_t k = 0; k < e->nz; ++k) E_CHECK(e, std::isfinite(lh[k]), "level heights must be finite");
+    const bool metres = lh[e->nz - 1] > 0.0f;
+    if (metres)
+        for (uint32_t k = 1; k < e->nz; ++k) E_CHECK(e, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
+    bool any = false;
+    for (char r : e->met_ready) any = any || r;
+    if (any) E_CHECK(e, std::equal(lh, lh + e->nz, e->lh_host.begin()), "members must share the level heights");
+    e->lh_host.assign(lh, lh + e->nz);
+    e->has_heights = metres;
+    e->zmax = metres ? lh[e->nz - 1] : (float)(e->nz - 1);
+    E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->nz * 4, hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, fp::launch_pack_met(u0, v0, w0, u1, v1, w1, e->met + (uint64_t)m * e->npts() * 2,
+                                 (uint32_t)e->npts(), e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    e->met_ready[m] = 1;
+    return FP_OK;
+}
+
+int fp_ens_pbl_upload(fp_ens *e, const float *ustar, const float *wstar, const float *hmix,
+                      const float *oli, const float *precip)
+{
+    E_CHECK(e, e && ustar && wstar && hmix && oli, "null argument");
+    const uint64_t np = (uint64_t)e->nx * e->ny;
+    const float *f[4] = {ustar, wstar, hmix, oli};
+    for (int k = 0; k < 4; ++k)
+        E_HIP(e, hipMemcpyAsync(e->pbl_stage + k * np, f[k], np * 4, hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, fp::launch_pack_pbl(e->pbl_stage, e->pbl, (uint32_t)np, e->stream));
+    if (precip) {
+        E_HIP(e, hipMemcpyAsync(e->precip, precip, np * 4, hipMemcpyHostToDevice, e->stream));
+        e->precip_ready = true;
+    }
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    e->pbl_ready = true;
+    return FP_OK;
+}
+
+int fp_ens_grid_configure(fp_ens *e, const fp_grid_params *p)
+{
+    E_CHECK(e, e && p, "null argument");
+    E_CHECK(e, e->grid_counts, "ensemble has no output grid");
+    E_CHECK(e, p->refine_x >= 1 && p->refine_y >= 1, "refine factors must be >= 1");
+    E_CHECK(e, p->species < e->n_species, "species index beyond n_species");
+    E_CHECK(e, std::isfinite(p->mass_scale), "mass_scale must be finite");
+    e->gp = *p;
+    e->grid_configured = true;
+    e->configured = false;  // launch tables hold the grid arguments
+    return FP_OK;
+}
+
+int fp_ens_grid_clear(fp_ens *e)
+{
+    E_CHECK(e, e && e->grid_counts, "ensemble has no output grid");
+    const uint64_t n = e->ncell() * e->n_members * 4;
+    E_HIP(e, hipMemsetAsync(e->grid_counts, 0, n, e->stream));
+    E_HIP(e, hipMemsetAsync(e->grid_mass, 0, n, e->stream));
+    return FP_OK;
+}
+
+int fp_ens_grid_download_member(fp_ens *e, uint32_t m, uint32_t *counts, uint32_t *mass)
+{
+    E_CHECK(e, e && e->grid_counts, "ensemble has no output grid");
+    E_CHECK(e, m < e->n_members, "member index out of range");
+    const uint64_t nc = e->ncell();
+    if (counts)
+        E_HIP(e, hipMemcpyAsync(counts, e->grid_counts + m * nc, nc * 4, hipMemcpyDeviceToHost, e->stream));
+    if (mass)
+        E_HIP(e, hipMemcpyAsync(mass, e->grid_mass + m * nc, nc * 4, hipMemcpyDeviceToHost, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+int fp_ens_set_launch(fp_ens *e, uint32_t block, uint32_t ppt)
+{
+    E_CHECK(e, e, "null ensemble");
+    E_CHECK(e, block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
+            "block_size must be 64, 128, 256, 512 or 1024");
+    E_CHECK(e, ppt >= 1 && ppt <= 16, "particles_per_thread must be 1..16");
+    e->block = block;
+    e->ppt = ppt;
+    e->configured = false;
+    drop_graph(e);
+    return FP_OK;
+}
+
+int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, const uint32_t *counters,
+                     const uint64_t *dispatch, double t0_in_bracket, double bracket_s)
+{
+    E_CHECK(e, e && p, "null argument");
+    E_STATE(e, e->pbl_ready, "fp_ens_configure before fp_ens_pbl_upload");
+    for (uint32_t m = 0; m < e->n_members; ++m) E_STATE(e, e->met_ready[m], "member met bracket missing");
+    E_CHECK(e, std::isfinite(p->dt_seconds) && p->dt_seconds > 0.0f, "invalid dt_seconds");
+    E_CHECK(e, p->n_substeps <= 4, "n_substeps must be 0..4");
+    E_CHECK(e, p->direction == 1 || p->direction == -1, "direction must be +1 or -1");
+    E_CHECK(e, p->dry_mode <= FP_DRY_UNIFORM, "ensembles support dry_mode off or uniform");
+    E_CHECK(e, p->wet_mode != FP_WET_PER_SLOT && p->wet_mode <= FP_WET_PRECIP_FIELD, "unsupported wet_mode");
+    E_STATE(e, p->wet_mode != FP_WET_PRECIP_FIELD || e->precip_ready, "precip field missing");
+    E_CHECK(e, p->loss_mode <= FP_LOSS_MIN_MASS, "unknown loss_mode");
+    E_STATE(e, !p->grid_fused || e->grid_configured, "fused gridding before fp_ens_grid_configure");
+    E_CHECK(e, bracket_s > 0.0 && std::isfinite(t0_in_bracket), "invalid bracket time");
+    drop_graph(e);
+    e->members.assign(e->n_members, fp::EnsMember{});
+    std::vector<fp::EnsBlock> blocks;
+    const uint64_t per_block = (uint64_t)e->block * e->ppt;
+    const uint64_t np = e->npts();
+    for (uint32_t m = 0; m < e->n_members; ++m) {
+        const uint64_t n = dispatch ? dispatch[m] : e->cap[m];
+        E_CHECK(e, n <= e->cap[m], "dispatch exceeds member capacity");
+        fp::StepArgs &a = e->members[m].a;
+        a.s = e->soa;
+        a.s.base += e->off[m];
+        a.n = (uint32_t)n;
+        a.n_species = e->n_species;
+        a.met = e->met + (uint64_t)m * np * 2;
+        a.nx = e->nx; a.ny = e->ny; a.nz = e->nz;
+        a.level_heights = e->level_heights;
+        a.has_heights = e->has_heights ? 1u : 0u;
+        a.zmax = e->zmax;
+        a.pbl = e->pbl;
+        a.precip = e->precip;
+        fp::fill_step_scalars(a, *p);
+        a.alpha = 0.0f;  // the ensemble clock supplies the temporal weight
+        if (keys) {
+            a.key0 = keys[2 * m];
+            a.key1 = keys[2 * m + 1];
+        }
+        if (e->grid_counts)
+            a.grid = fp::make_grid_args(e->grid_counts + (uint64_t)m * e->ncell(),
+                                        e->grid_mass + (uint64_t)m * e->ncell(), e->ox, e->oy, e->oz, e->gp);
+        a.ppt = e->ppt;
+        for (int k = 0; k < 4; ++k) e->members[m].base_ctr[k] = counters ? counters[4 * m + k] : p->counter[k];
+        e->members[m].ctr_per_step = fp::counter_per_step(n, p->n_substeps);
+        for (uint64_t f = 0; f < n; f += per_block) blocks.push_back(fp::EnsBlock{m, (uint32_t)f});
+    }
+    e->nblocks = (uint32_t)blocks.size();
+    if (blocks.size() > e->d_blocks_cap) {
+        if (e->d_blocks) (void)hipFree(e->d_blocks);
+        e->d_blocks = nullptr;
+        e->d_blocks_cap = 0;
+        E_HIP(e, dmalloc(&e->d_blocks, blocks.size()));
+        e->d_blocks_cap = blocks.size();
+    }
+    fp::EnsClock clk{};
+    clk.next_step = 0;
+    clk.t0_in_bracket = t0_in_bracket;
+    clk.dt_seconds = p->direction < 0 ? -(double)p->dt_seconds : (double)p->dt_seconds;
+    clk.bracket_seconds = bracket_s;
+    E_HIP(e, hipMemcpyAsync(e->d_members, e->members.data(), e->members.size() * sizeof(fp::EnsMember),
+                            hipMemcpyHostToDevice, e->stream));
+    if (!blocks.empty())
+        E_HIP(e, hipMemcpyAsync(e->d_blocks, blocks.data(), blocks.size() * sizeof(fp::EnsBlock),
+                                hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, hipMemcpyAsync(e->d_clock, &clk, sizeof clk, hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    e->sub4 = p->n_substeps == 4;
+    e->steps_done = 0;
+    e->configured = true;
+    return FP_OK;
+}
+
+int fp_ens_step(fp_ens *e, uint32_t n_steps, int use_graph, uint32_t *counters_out)
+{
+    E_CHECK(e, e, "null ensemble");
+    E_STATE(e, e->configured, "fp_ens_step before fp_ens_configure");
+    if (n_steps == 0) return FP_OK;
+    if (use_graph && !e->exec) {
+        E_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t h = fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block,
+                                                 e->sub4, e->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t h2 = hipStreamEndCapture(e->stream, &g);
+        E_HIP(e, h);
+        E_HIP(e, h2);
+        e->graph = g;
+        E_HIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    }
+    E_HIP(e, hipEventRecord(e->ev_a, e->stream));
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        if (use_graph) E_HIP(e, hipGraphLaunch(e->exec, e->stream));
+        else E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->sub4,
+                                          e->stream));
+    }
+    E_HIP(e, hipEventRecord(e->ev_b, e->stream));
+    E_HIP(e, hipEventSynchronize(e->ev_b));
+    float ms = 0.0f;
+    E_HIP(e, hipEventElapsedTime(&ms, e->ev_a, e->ev_b));
+    e->last_ms = ms / n_steps;
+    e->steps_done += n_steps;
+    if (counters_out)
+        for (uint32_t m = 0; m < e->n_members; ++m)
+            fp::counter_advance(e->members[m].base_ctr, e->steps_done * e->members[m].ctr_per_step,
+                                counters_out + 4 * m);
+    return FP_OK;
+}
+
+int fp_ens_last_step_ms(fp_ens *e, double *ms)
+{
+    E_CHECK(e, e && ms, "null argument");
+    *ms = e->last_ms;
+    return FP_OK;
+}
+
+}  // extern "C"

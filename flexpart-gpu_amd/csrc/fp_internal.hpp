@@ -87,6 +87,20 @@ struct StepArgs {
     double log_1em3;        // fpm_log_d(1e-3f): the log of the unstable profile's zeta floor
 };
 
+// Ensemble launch tables (device memory).
+struct EnsMember {
+    StepArgs a;              // member's particle range, met, grid, key, scalars
+    uint32_t base_ctr[4];    // Philox counter at clock step 0
+    uint64_t ctr_per_step;   // dispatch * blocks per particle
+};
+struct EnsBlock { uint32_t member; uint32_t first; };  // first member-local slot
+struct EnsClock {
+    uint32_t next_step, step;
+    float alpha;
+    uint32_t pad;
+    double t0_in_bracket, dt_seconds, bracket_seconds;
+};
+
 // Which of the two SoA sets holds the current data of each slot range.
 // Compaction and sort write the dispatch range into the other set and swap;
 // slots beyond the dispatch range keep their data in the set that last wrote
@@ -193,6 +207,8 @@ namespace fp {
 
 // Kernel launchers (fp_kernels.hip).
 hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
+hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
+                           uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st);
 hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
                            const float *v1, const float *w1, float4 *met, uint32_t npts,
                            hipStream_t st);
@@ -210,5 +226,19 @@ hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out,
                              hipStream_t st);
 // Morton sort: keys + stable radix sort + gather (fp_sort.hip).
 hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n);
+
+// Shared by the single-run and ensemble entry points (fp_api.hip).
+// Scalar step arguments (timestep, scales, Philox key and counter, sub-steps,
+// deposition and loss modes) from the public parameter block.
+void fill_step_scalars(StepArgs &a, const fp_step_params &p);
+GridArgs make_grid_args(uint32_t *counts, uint32_t *mass_fp, uint32_t nx, uint32_t ny, uint32_t nz,
+                        const fp_grid_params &gp);
+// 128-bit little-endian counter + off (gpu/langevin_fused.rs:229-233).
+void counter_advance(const uint32_t c[4], uint64_t off, uint32_t out[4]);
+// Philox counter blocks one step consumes: dispatch * blocks per particle.
+inline uint64_t counter_per_step(uint64_t dispatch, uint32_t n_substeps)
+{
+    return dispatch * (n_substeps <= 2 ? 1ull : 2ull);
+}
 
 }  // namespace fp

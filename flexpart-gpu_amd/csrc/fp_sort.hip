@@ -62,22 +62,35 @@ __global__ void __launch_bounds__(kRB) bounds_partial_kernel(Soa s, uint32_t n, 
     }
 }
 
-__global__ void bounds_final_kernel(const double *part, uint32_t nparts, double *bounds)
+// One block folds the per-block partials (min/max are exact, so the order
+// of reduction does not matter).
+__global__ void __launch_bounds__(kRB) bounds_final_kernel(const double *part, uint32_t nparts,
+                                                           double *bounds)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
-    double any = 0.0;
-    for (uint32_t b = 0; b < nparts; ++b) {
+    __shared__ double red[7][kRB];
+    double v[7] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    for (uint32_t b = threadIdx.x; b < nparts; b += kRB) {
         const double *p = part + b * 7;
         if (p[6] == 0.0) continue;
-        any = 1.0;
+        v[6] = 1.0;
         for (int k = 0; k < 6; k += 2) {
             v[k] = fmin(v[k], p[k]);
             v[k + 1] = fmax(v[k + 1], p[k + 1]);
         }
     }
-    for (int k = 0; k < 6; ++k) bounds[k] = v[k];
-    bounds[6] = any;
+    for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (uint32_t off = kRB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int k = 0; k < 6; k += 2) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + off]);
+                red[k + 1][threadIdx.x] = fmax(red[k + 1][threadIdx.x], red[k + 1][threadIdx.x + off]);
+            }
+            red[6][threadIdx.x] = fmax(red[6][threadIdx.x], red[6][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) bounds[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // quantize_axis (particles/mod.rs:415-425)
@@ -124,13 +137,24 @@ __global__ void keys_kernel(Soa s, uint32_t n, const double *bounds, uint32_t bi
     vals[i] = (uint32_t)i;
 }
 
-__global__ void gather_kernel(Soa in, Soa out, uint32_t n, const uint32_t *new_to_old)
+// Record gather: the permutation reads whole 96-byte particle records from an
+// AoS copy (six 16-byte loads per slot, no partial cache lines wasted on
+// random 4-byte reads) and writes each field of the sorted SoA coalesced.
+__global__ void gather_records_kernel(const uint4 *aos, Soa out, uint32_t n, const uint32_t *new_to_old)
 {
     const uint64_t j = (uint64_t)blockIdx.x * kRB + threadIdx.x;
     if (j >= n) return;
-    const uint64_t src = new_to_old[j];
+    const uint4 *rec = aos + (uint64_t)new_to_old[j] * (F_COUNT / 4);
+    uint4 w[F_COUNT / 4];
 #pragma unroll
-    for (uint32_t f = 0; f < F_COUNT; ++f) out.u(f)[j] = in.u(f)[src];
+    for (uint32_t q = 0; q < F_COUNT / 4; ++q) w[q] = rec[q];
+#pragma unroll
+    for (uint32_t q = 0; q < F_COUNT / 4; ++q) {
+        out.u(4 * q + 0)[j] = w[q].x;
+        out.u(4 * q + 1)[j] = w[q].y;
+        out.u(4 * q + 2)[j] = w[q].z;
+        out.u(4 * q + 3)[j] = w[q].w;
+    }
 }
 
 template <typename K>
@@ -174,7 +198,7 @@ hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n)
     } else {
         const uint32_t nparts = (uint32_t)std::min<uint64_t>((n + kRB - 1) / kRB, 2048);
         bounds_partial_kernel<<<nparts, kRB, 0, st>>>(in, n, ctx->bounds_partial);
-        bounds_final_kernel<<<1, 64, 0, st>>>(ctx->bounds_partial, nparts, ctx->bounds_dev);
+        bounds_final_kernel<<<1, kRB, 0, st>>>(ctx->bounds_partial, nparts, ctx->bounds_dev);
     }
     const uint32_t grid = (n + kRB - 1) / kRB;
     const uint32_t end_bit = 3u * p.bits_per_axis + 1u;
@@ -190,7 +214,11 @@ hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n)
         e = sort_pairs<uint64_t>(ctx, n, end_bit);
     }
     if (e != hipSuccess) return e;
-    gather_kernel<<<grid, kRB, 0, st>>>(in, out, n, ctx->sort_vals_alt);
+    // ctx->aos_stage holds at least n records (ensured by the caller)
+    e = launch_soa_to_aos(in, ctx->aos_stage, n, st);
+    if (e != hipSuccess) return e;
+    gather_records_kernel<<<grid, kRB, 0, st>>>(reinterpret_cast<const uint4 *>(ctx->aos_stage), out, n,
+                                                ctx->sort_vals_alt);
     return hipGetLastError();
 }
 

@@ -119,7 +119,8 @@ struct Wind { float u, v, w; };
 // sample_dual_u/v/w (advection_dual_wind.wgsl:142-236): trilinear per time
 // level, then the temporal blend.  The 8 corners come from packed 32-byte
 // records {u0,v0,w0,u1,v1,w1,-,-}: two 16-byte loads per corner.
-__device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, float x, float y, float z)
+__device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, float alpha, float x,
+                                            float y, float z)
 {
     const Axis ax = axis_of(x, a.nx);
     const Axis ay = axis_of(y, a.ny);
@@ -137,7 +138,7 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
         c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
         c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
     }
-    const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - a.alpha;
+    const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - alpha;
     float v[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
@@ -145,7 +146,7 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
         const float x2 = mix(c[4][f], c[5][f], ax.t, ux), x3 = mix(c[6][f], c[7][f], ax.t, ux);
         v[f] = mix(mix(x0, x1, ay.t, uy), mix(x2, x3, ay.t, uy), az.t, uz);
     }
-    return Wind{mix(v[0], v[3], a.alpha, ua), mix(v[1], v[4], a.alpha, ua), mix(v[2], v[5], a.alpha, ua)};
+    return Wind{mix(v[0], v[3], alpha, ua), mix(v[1], v[4], alpha, ua), mix(v[2], v[5], alpha, ua)};
 }
 
 __device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
@@ -418,24 +419,22 @@ __device__ __forceinline__ void grid_reduce_wave(const GridArgs &g, bool valid, 
 // The kernel is latency-bound at low occupancy; the register budget is
 // capped for 3 wavefronts per SIMD (<= 168 VGPRs), the most that fits
 // without spilling to scratch.
-template <uint32_t BLOCK, int NSUB>
-__global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
+// XCD-aware chunking: the hardware deals blocks round-robin over the 8
+// XCDs; give each XCD a contiguous run of particle chunks so that
+// Morton-neighbouring particles share one L2.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb)
 {
-    __shared__ float lh[kMaxLevels];
-    __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
-    for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
-    if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = a.grid.outheights[threadIdx.x];
-    __syncthreads();
-
-    // XCD-aware chunking: the hardware deals blocks round-robin over the 8
-    // XCDs; give each XCD a contiguous run of particle chunks so that
-    // Morton-neighbouring particles share one L2.
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t xcd = b % kXcds, k = b / kXcds, per = nb / kXcds, rem = nb % kXcds;
-    const uint32_t chunk = xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
-    const uint64_t first = (uint64_t)chunk * BLOCK * a.ppt;
-    const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
+    return xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+}
 
+// One block's chunk of the step: slots first + r*BLOCK + threadIdx.x for
+// r < a.ppt, temporal weight alpha and Philox base counter ctr.
+template <uint32_t BLOCK, int NSUB>
+__device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, float alpha, U4 ctr,
+                                           const float *lh, const float *oh)
+{
+    const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
     const Soa s = a.s;
     for (uint32_t r = 0; r < a.ppt; ++r) {
         const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
@@ -458,12 +457,12 @@ __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
             // ---- advection (advection_dual_wind.wgsl:254-303)
             const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
             const float x0 = (float)cx + px, y0 = (float)cy + py, z0 = pz;
-            const Wind s0 = sample_wind(a, lh, x0, y0, z0);
+            const Wind s0 = sample_wind(a, lh, alpha, x0, y0, z0);
             const float u0 = s0.u + tu, v0 = s0.v + tv, w0 = s0.w;
             const float qx = clampf(x0 + (u0 * a.x_scale) * a.dt_adv, 0.0f, nxm);
             const float qy = clampf(y0 + (v0 * a.y_scale) * a.dt_adv, 0.0f, nym);
             const float qz = clampf(z0 + (w0 * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
-            const Wind s1 = sample_wind(a, lh, qx, qy, qz);
+            const Wind s1 = sample_wind(a, lh, alpha, qx, qy, qz);
             const float u1 = s1.u + tu, v1 = s1.v + tv, w1 = s1.w;
             const float cu = 0.5f * (u0 + u1), cv = 0.5f * (v0 + v1), cw = 0.5f * (w0 + w1);
             const float x1 = clampf(x0 + (cu * a.x_scale) * a.dt_adv, 0.0f, nxm);
@@ -483,10 +482,10 @@ __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
 
             // ---- Philox blocks per slot (langevin_fused.wgsl:430-451)
             const uint64_t cbase = i * a.blocks_per_particle;
-            const U4 b0 = philox10(ctr_add(a.c0, a.c1, a.c2, a.c3, cbase), a.key0, a.key1);
+            const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
             U4 b1 = U4{0u, 0u, 0u, 0u};
             if (a.blocks_per_particle > 1u)
-                b1 = philox10(ctr_add(a.c0, a.c1, a.c2, a.c3, cbase + 1u), a.key0, a.key1);
+                b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
             float eu, ev, ew0, ew1, ew2, ew3;
             gauss_pair(b0.x, b0.y, eu, ev);
             gauss_pair(b0.z, b0.w, ew0, ew1);
@@ -577,6 +576,58 @@ __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
 }
 
 template <uint32_t BLOCK>
+__device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float *oh)
+{
+    for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
+    if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = a.grid.outheights[threadIdx.x];
+    __syncthreads();
+}
+
+template <uint32_t BLOCK, int NSUB>
+__global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
+{
+    __shared__ float lh[kMaxLevels];
+    __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
+    load_tables<BLOCK>(a, lh, oh);
+    const uint64_t first = (uint64_t)xcd_chunk(blockIdx.x, gridDim.x) * BLOCK * a.ppt;
+    step_chunk<BLOCK, NSUB>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh);
+}
+
+// ---------------------------------------------------------------- ensembles
+//
+// All members of an ensemble advance in one launch: a block table maps each
+// block to (member, first slot); every member has its own particle range,
+// met bracket, Philox key/counter and output grid (EnsMember, device
+// memory), and they share the step clock, so one launch (and one captured
+// graph node) covers members of any size.
+
+__global__ void ens_clock_kernel(EnsClock *c)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t k = c->next_step;
+    c->step = k;
+    // interpolation_alpha (timeloop.rs:2086-2120): f64, then f32, clamped
+    const double t = c->t0_in_bracket + (double)k * c->dt_seconds;
+    const float alpha = (float)(t / c->bracket_seconds);
+    c->alpha = fminf(fmaxf(alpha, 0.0f), 1.0f);
+    c->next_step = k + 1u;
+}
+
+template <uint32_t BLOCK, int NSUB>
+__global__ void __launch_bounds__(BLOCK, 3) ens_step_kernel(const EnsMember *members,
+                                                            const EnsBlock *blocks, const EnsClock *clock)
+{
+    __shared__ float lh[kMaxLevels];
+    __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
+    load_tables<BLOCK>(members[0].a, lh, oh);  // one vertical grid for all members
+    const EnsBlock eb = blocks[xcd_chunk(blockIdx.x, gridDim.x)];
+    const EnsMember &m = members[eb.member];
+    const U4 ctr = ctr_add(m.base_ctr[0], m.base_ctr[1], m.base_ctr[2], m.base_ctr[3],
+                           (uint64_t)clock->step * m.ctr_per_step);
+    step_chunk<BLOCK, NSUB>(m.a, eb.first, clock->alpha, ctr, lh, oh);
+}
+
+template <uint32_t BLOCK>
 __global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs g)
 {
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
@@ -613,6 +664,30 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st)
     case 256: return launch_block<256>(a, grid, st);
     case 512: return launch_block<512>(a, grid, st);
     case 1024: return launch_block<1024>(a, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <uint32_t BLOCK>
+hipError_t launch_ens_block(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
+                            uint32_t nblocks, bool sub4, hipStream_t st)
+{
+    ens_clock_kernel<<<1, 64, 0, st>>>(clock);
+    if (nblocks == 0) return hipGetLastError();
+    if (sub4) ens_step_kernel<BLOCK, 4><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
+    else ens_step_kernel<BLOCK, -1><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
+                           uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st)
+{
+    switch (block) {
+    case 64: return launch_ens_block<64>(members, blocks, clock, nblocks, sub4, st);
+    case 128: return launch_ens_block<128>(members, blocks, clock, nblocks, sub4, st);
+    case 256: return launch_ens_block<256>(members, blocks, clock, nblocks, sub4, st);
+    case 512: return launch_ens_block<512>(members, blocks, clock, nblocks, sub4, st);
+    case 1024: return launch_ens_block<1024>(members, blocks, clock, nblocks, sub4, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -244,6 +244,89 @@ int fp_profile_reset(fp_ctx *ctx);
  * block size (64..1024, multiple of 64) and particles per thread (1..16). */
 int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_thread);
 
+/* Autotuner (gpu/workgroup.rs:133-147,342-445: candidate sweep, warm-up runs,
+ * median of measured runs, deterministic tie-break, per-device JSON cache),
+ * extended to particles per thread and the Morton re-sort interval.  Trials
+ * run fp_step (and fp_sort_morton) on a snapshot of the current particle
+ * state and output grid, which are restored afterwards.  The chosen launch
+ * shape is applied to the context; the re-sort interval is returned for the
+ * caller's time loop (ForwardSpatialSortConfig::interval_steps). */
+typedef struct fp_autotune_params {
+    uint32_t warmup_steps;        /* per candidate (reference: 2) */
+    uint32_t measured_steps;      /* per candidate, median taken (reference: 7) */
+    uint32_t sort_bits;           /* bits_per_axis for re-sort trials */
+    uint32_t n_sort_intervals;    /* candidates in sort_intervals (0 = keep the current) */
+    uint32_t sort_intervals[8];   /* steps between re-sorts; 0 = never */
+    uint32_t sort_trial_steps;    /* steps timed per re-sort candidate */
+    const char *cache_path;       /* JSON cache file, NULL = no cache */
+} fp_autotune_params;
+
+typedef struct fp_autotune_result {
+    uint32_t block_size;
+    uint32_t particles_per_thread;
+    uint32_t sort_interval;
+    uint32_t from_cache;
+    double step_ms;               /* median step time of the chosen launch shape */
+    double step_ms_with_sort;     /* mean step time including amortised re-sorts */
+} fp_autotune_result;
+
+int fp_autotune(fp_ctx *ctx, const fp_step_params *base, const fp_autotune_params *params,
+                fp_autotune_result *out);
+
+/* ---- batched ensembles -------------------------------------------------
+ * An ensemble runs many independent simulations (members) of unequal
+ * particle counts on one vertical grid, each with its own particle range,
+ * met bracket, Philox key and counter and output grid, and a shared PBL
+ * state.  One launch per sync step advances every member (fp_ens_step),
+ * optionally replayed from a captured hipGraph.  Each member's result is
+ * exactly what fp_step gives for that member alone.  No counterpart in the
+ * reference, which runs one simulation per driver. */
+typedef struct fp_ens fp_ens;
+
+typedef struct fp_ens_desc {
+    uint32_t n_members;
+    uint32_t met_nx, met_ny, met_nz;
+    uint32_t n_species;
+    uint32_t out_nx, out_ny, out_nz;  /* per-member output grid (0 = none) */
+    const uint64_t *capacities;       /* n_members particle capacities */
+} fp_ens_desc;
+
+int fp_ens_create(int device_ordinal, const fp_ens_desc *desc, fp_ens **out);
+int fp_ens_destroy(fp_ens *ens);
+const char *fp_ens_last_error(const fp_ens *ens);
+int fp_ens_sync(fp_ens *ens);
+/* member slots [0, n) from AoS records; n <= that member's capacity */
+int fp_ens_upload_member(fp_ens *ens, uint32_t member, const fp_particle *p, uint64_t n);
+int fp_ens_download_member(fp_ens *ens, uint32_t member, fp_particle *out, uint64_t n);
+/* member met bracket from device pointers (layout of fp_met_upload_bracket);
+ * the level heights must be the same for every member */
+int fp_ens_met_upload_member_dev(fp_ens *ens, uint32_t member, const float *u0, const float *v0,
+                                 const float *w0, const float *u1, const float *v1,
+                                 const float *w1, const float *level_heights_host);
+int fp_ens_pbl_upload(fp_ens *ens, const float *ustar, const float *wstar, const float *hmix,
+                      const float *oli, const float *precip);
+int fp_ens_grid_configure(fp_ens *ens, const fp_grid_params *params);
+int fp_ens_grid_clear(fp_ens *ens);
+int fp_ens_grid_download_member(fp_ens *ens, uint32_t member, uint32_t *counts, uint32_t *mass_fp);
+/* Step parameters for the next fp_ens_step calls: base holds everything but
+ * the per-member Philox state (keys: 2 per member, NULL = base->key;
+ * counters: 4 per member at the first step) and the temporal weight, which
+ * is (time_in_bracket + k * direction * dt) / bracket_seconds, clamped to
+ * [0, 1], at clock step k (k = 0 at the first step after this call).
+ * dispatch: per-member slot counts (NULL = capacities).  Rebuilds the
+ * launch table and drops a captured graph. */
+int fp_ens_configure(fp_ens *ens, const fp_step_params *base, const uint32_t *keys,
+                     const uint32_t *counters, const uint64_t *dispatch, double time_in_bracket_s,
+                     double bracket_s);
+/* n_steps steps of every member; use_graph = 1 captures the step once
+ * (hipGraph) and replays it.  counters_out (4 per member, may be NULL)
+ * receives each member's counter after the last step. */
+int fp_ens_step(fp_ens *ens, uint32_t n_steps, int use_graph, uint32_t *counters_out);
+int fp_ens_set_launch(fp_ens *ens, uint32_t block_size, uint32_t particles_per_thread);
+/* mean milliseconds per fp_ens step launch over the last fp_ens_step call
+ * (HIP events on the ensemble stream). Blocking. */
+int fp_ens_last_step_ms(fp_ens *ens, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
