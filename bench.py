@@ -84,7 +84,8 @@ def cpu_baseline(w, threads: int, sample_every: int = 5, steps: int = 4):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=144, help="timed steps (144 = the 24 h run)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 144 = the 24 h run; 36 = 6 h for the ensemble)")
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--particles", type=int, default=10_000_000)
     ap.add_argument("--sort-every", type=int, default=-1,
@@ -93,6 +94,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
     ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("global_10m", "ensemble"), default="global_10m",
+                    help="global_10m (the headline line) or the 96-member batched ensemble")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -101,6 +104,11 @@ def main(argv=None) -> int:
               "exchange inside a step, so only --gpus 1 is supported in this build",
               file=sys.stderr, flush=True)
         return 2
+
+    if args.workload == "ensemble":
+        args.steps = 36 if args.steps is None else args.steps
+        return bench_ensemble(args)
+    args.steps = 144 if args.steps is None else args.steps
 
     import numpy as np
     import torch
@@ -224,6 +232,187 @@ def main(argv=None) -> int:
     if not args.no_cpu_baseline:
         threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
         result["cpu_baseline"] = cpu_baseline(w, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_ensemble(args) -> int:
+    """96 members (lognormal sizes, median 200k, about 32M particles) released
+    over the first 6 h, each with its own 10 % wind perturbation of the global
+    360 x 181 x 60 met, its own Philox key and its own 360 x 181 x 5 grid.
+    After the releases (untimed) and an untimed launch-shape sweep, the same
+    36-step window (6 h, 600 s steps) is timed three ways: a loop over
+    per-member contexts (fp_step per member, each on its own stream), the
+    ensemble with one direct launch per step, and the ensemble replaying its
+    captured graph."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    spec = W.ensemble_spec()
+    M = len(spec.counts)
+    caps = spec.counts.astype(np.uint64)
+    total = int(caps.sum())
+    met = W.global_met(2, n_levels=3)
+    pbl = W.global_pbl(2)
+    nx, ny, nz = met.shape
+    out_shape = (nx, ny, 5)
+    outheights = (100.0, 500.0, 1000.0, 3000.0, 10_000.0)
+    bracket_s = met.seconds_between_levels
+    dt = spec.dt_seconds
+    window = int(round(bracket_s / dt))
+    dev = torch.device("cuda:0")
+    levels = [[torch.from_numpy(np.ascontiguousarray(a)).to(dev).view(nx, ny, nz) for a in lvl]
+              for lvl in met.levels]
+    factors = [torch.from_numpy(W.ensemble_member_factor(spec, m, nx, ny)).to(dev)[:, :, None]
+               for m in range(M)]
+    torch.cuda.synchronize()
+
+    ens = fp.GpuEnsemble(caps, met.shape, n_species=1, out_shape=out_shape)
+    loop = [fp.GpuTimestep(int(caps[m]), met.shape, n_species=1, out_shape=out_shape) for m in range(M)]
+    for c in [ens] + loop:
+        c.upload_pbl(pbl.ustar, pbl.wstar, pbl.hmix, pbl.oli)
+        c.configure_grid(outheights, (1, 1), 0, 1.0e8)
+        c.clear_grid()
+
+    def upload_bracket(b):
+        for m in range(M):
+            ts = [(levels[b + t][c] * factors[m]).contiguous() for t in (0, 1) for c in range(3)]
+            torch.cuda.synchronize()
+            ptrs = [t.data_ptr() for t in ts]
+            ens.upload_met_member_dev(m, ptrs, met.level_heights)
+            loop[m].upload_met_bracket_dev(ptrs, met.level_heights)
+            del ts
+
+    cfg = fp.StepConfig(dt_seconds=dt, x_scale=1.0 / W.METRES_PER_DEG, y_scale=1.0 / W.METRES_PER_DEG,
+                        dry_mode=1, dry_vd=2e-3, grid_fused=True)
+    keys = np.array(spec.keys, np.uint32)
+    alpha = [float(np.float32((k * dt) / bracket_s)) for k in range(window + 1)]
+
+    # releases over the first bracket (untimed): a member joins the launch
+    # table at its release step; the loop steps released members only
+    t_gen = time.perf_counter()
+    upload_bracket(0)
+    ens_ctr = np.zeros((M, 4), np.uint32)
+    loop_ctr = [(0, 0, 0, 0)] * M
+    released = np.zeros(M, bool)
+    for k in range(window):
+        newly = np.nonzero(spec.start_step == k)[0]
+        for m in newly:
+            p = W.ensemble_member_particles(spec, int(m))
+            ens.upload_member(int(m), p)
+            loop[m].upload_store(p)
+            released[m] = True
+        if len(newly):
+            ens.configure(cfg, keys=keys, counters=ens_ctr, dispatch=np.where(released, caps, 0),
+                          time_in_bracket_s=k * dt, bracket_s=bracket_s)
+        ens_ctr = ens.step(1)
+        for m in np.nonzero(released)[0]:
+            c = fp.StepConfig(**{**cfg.__dict__, "alpha": alpha[k], "key": tuple(keys[m]),
+                                 "counter": loop_ctr[m]})
+            loop_ctr[m] = loop[m].step(c)
+    assert released.all()
+    for m in (0, int(np.argmax(caps)), int(np.argmin(caps))):
+        a, b = ens.download_member(m), loop[m].download_particles()
+        if a.tobytes() != b.tobytes():
+            raise RuntimeError(f"ensemble member {m} differs from its single-run context")
+    upload_bracket(1)
+    print(f"# ensemble: {M} members, {total} particles, releases done in "
+          f"{time.perf_counter() - t_gen:.1f} s; members equal single runs", file=sys.stderr, flush=True)
+
+    # launch-shape sweep on the ensemble's graph path (untimed)
+    best = None
+    for block in (128, 256, 512):
+        for ppt in (1, 2, 4):
+            ens.set_launch(block, ppt)
+            ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
+            ens_ctr = ens.step(2, use_graph=True)
+            ens_ctr = ens.step(6, use_graph=True)
+            ms = ens.last_step_ms()
+            if best is None or ms < best[0]:
+                best = (ms, block, ppt)
+    _, block, ppt = best
+    ens.set_launch(block, ppt)
+    for c in loop:
+        c.set_launch(block, ppt)
+    print(f"# ensemble launch sweep: block {block}, {ppt} particles/thread, {best[0]:.3f} ms/step",
+          file=sys.stderr, flush=True)
+
+    # (a) loop over members: one fp_step per member per step, parameter
+    # blocks prepared up front so the host loop is ctypes calls only
+    lib = fp.library()
+    params = []
+    for m in range(M):
+        p = fp.StepConfig(**{**cfg.__dict__, "key": tuple(keys[m]), "counter": loop_ctr[m]}).to_c()
+        params.append((loop[m]._h, p, (C.c_uint32 * 4)()))
+    for c in loop:
+        c.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        a = alpha[min(k, window)]
+        for h, p, nxt in params:
+            p.alpha = a
+            if lib.fp_step(h, C.byref(p), nxt) != 0:
+                raise RuntimeError(lib.fp_last_error(h).decode())
+            C.memmove(p.counter, nxt, 16)
+    for c in loop:
+        c.sync()
+    t_loop = time.perf_counter() - t0
+
+    # (b) ensemble, one direct launch per step; (c) the captured graph
+    ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens_ctr = ens.step(args.steps, use_graph=False)
+    t_direct = time.perf_counter() - t0
+    ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
+    ens_ctr = ens.step(1, use_graph=True)  # capture
+    ens.sync()
+    t0 = time.perf_counter()
+    ens_ctr = ens.step(args.steps, use_graph=True)
+    t_graph = time.perf_counter() - t0
+    step_ms = ens.last_step_ms()
+
+    ps = total * args.steps
+    achieved = total * BYTES_PER_PARTICLE_STEP / (step_ms / 1e3) / 1e9
+    result = {
+        "metric": "ensemble particle-steps/s (96 members, 1 GPU)",
+        "value": ps / t_graph,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": t_graph * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "ensemble_96: 96 members, lognormal sizes (median 200k, clipped 10k-4M), "
+                        "released U[0,6) h, per-member 10% wind perturbation of the 360x181x60 "
+                        "met, per-member 360x181x5 fused grids, 600 s step, 4 vertical sub-steps",
+            "members": M,
+            "particles": total,
+            "block": block,
+            "particles_per_thread": ppt,
+            "loop_particle_steps_per_s": ps / t_loop,
+            "direct_particle_steps_per_s": ps / t_direct,
+            "graph_particle_steps_per_s": ps / t_graph,
+            "speedup_graph_vs_loop": t_loop / t_graph,
+            "speedup_direct_vs_loop": t_loop / t_direct,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    for c in loop:
+        c.close()
+    ens.close()
     print(json.dumps(result), flush=True)
     return 0
 

@@ -160,6 +160,7 @@ ABI = {
     "fp_ens_sync": (C.c_int, [_vp]),
     "fp_ens_upload_member": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),
     "fp_ens_download_member": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),
+    "fp_ens_met_upload_member": (C.c_int, [_vp, C.c_uint32, _f, _f, _f, _f, _f, _f, _f]),
     "fp_ens_met_upload_member_dev": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
     "fp_ens_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
     "fp_ens_grid_configure": (C.c_int, [_vp, C.POINTER(GridParams)]),
@@ -521,6 +522,13 @@ class GpuEnsemble:
         out = np.zeros(n, PARTICLE_DTYPE)
         self._chk(self._lib.fp_ens_download_member(self._h, member, out.ctypes.data, n))
         return out
+
+    def upload_met_member(self, member: int, u0, v0, w0, u1, v1, w1, level_heights):
+        n = int(np.prod(self.met_shape))
+        arrs = [_f32(a, n) for a in (u0, v0, w0, u1, v1, w1)]
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_ens_met_upload_member(self._h, member, *[_fp(a) for a in arrs],
+                                                     _fp(lh)))
 
     def upload_met_member_dev(self, member: int, ptrs, level_heights):
         lh = _f32(level_heights, self.met_shape[2])

@@ -23,6 +23,7 @@ struct fp_ens {
     fp::Soa soa;
 
     float4 *met = nullptr;  // n_members * npts * 2
+    float *met_stage = nullptr;  // 6 * npts floats for host uploads
     float *level_heights = nullptr;
     std::vector<float> lh_host;
     bool has_heights = false;
@@ -107,13 +108,46 @@ void destroy(fp_ens *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
-    fr(e->soa.base); fr(e->met); fr(e->level_heights); fr(e->pbl); fr(e->precip); fr(e->pbl_stage);
+    fr(e->soa.base); fr(e->met); fr(e->met_stage); fr(e->level_heights); fr(e->pbl); fr(e->precip); fr(e->pbl_stage);
     fr(e->grid_counts); fr(e->grid_mass); fr(e->aos_stage); fr(e->d_members); fr(e->d_blocks);
     fr(e->d_clock);
     if (e->ev_a) (void)hipEventDestroy(e->ev_a);
     if (e->ev_b) (void)hipEventDestroy(e->ev_b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+}
+
+int met_member_common(fp_ens *e, uint32_t m, const float *const f[6], hipMemcpyKind kind, const float *lh)
+{
+    E_CHECK(e, m < e->n_members, "member index out of range");
+    for (uint32_t k = 0; k < e->nz; ++k) E_CHECK(e, std::isfinite(lh[k]), "level heights must be finite");
+    const bool metres = lh[e->nz - 1] > 0.0f;
+    if (metres)
+        for (uint32_t k = 1; k < e->nz; ++k) E_CHECK(e, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
+    bool any = false;
+    for (char r : e->met_ready) any = any || r;
+    if (any) E_CHECK(e, std::equal(lh, lh + e->nz, e->lh_host.begin()), "members must share the level heights");
+    e->lh_host.assign(lh, lh + e->nz);
+    e->has_heights = metres;
+    e->zmax = metres ? lh[e->nz - 1] : (float)(e->nz - 1);
+    E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->nz * 4, hipMemcpyHostToDevice, e->stream));
+    const uint64_t np = e->npts();
+    const float *src[6];
+    if (kind == hipMemcpyHostToDevice) {
+        // stage the six host fields, then pack on the device
+        if (!e->met_stage) E_HIP(e, dmalloc(&e->met_stage, np * 6));
+        for (int k = 0; k < 6; ++k) {
+            E_HIP(e, hipMemcpyAsync(e->met_stage + k * np, f[k], np * 4, hipMemcpyHostToDevice, e->stream));
+            src[k] = e->met_stage + k * np;
+        }
+    } else {
+        for (int k = 0; k < 6; ++k) src[k] = f[k];
+    }
+    E_HIP(e, fp::launch_pack_met(src[0], src[1], src[2], src[3], src[4], src[5], e->met + (uint64_t)m * np * 2,
+                                 (uint32_t)np, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    e->met_ready[m] = 1;
+    return FP_OK;
 }
 
 }  // namespace
@@ -240,27 +274,20 @@ int fp_ens_download_member(fp_ens *e, uint32_t m, fp_particle *out, uint64_t n)
     return FP_OK;
 }
 
+int fp_ens_met_upload_member(fp_ens *e, uint32_t m, const float *u0, const float *v0, const float *w0,
+                             const float *u1, const float *v1, const float *w1, const float *lh)
+{
+    E_CHECK(e, e && u0 && v0 && w0 && u1 && v1 && w1 && lh, "null argument");
+    const float *f[6] = {u0, v0, w0, u1, v1, w1};
+    return met_member_common(e, m, f, hipMemcpyHostToDevice, lh);
+}
+
 int fp_ens_met_upload_member_dev(fp_ens *e, uint32_t m, const float *u0, const float *v0, const float *w0,
                                  const float *u1, const float *v1, const float *w1, const float *lh)
 {
     E_CHECK(e, e && u0 && v0 && w0 && u1 && v1 && w1 && lh, "null argument");
-    E_CHECK(e, m < e->n_members, "member index out of range");
-    for (uint32_t k = 0; k < e->nz; ++k) E_CHECK(e, std::isfinite(lh[k]), "level heights must be finite");
-    const bool metres = lh[e->nz - 1] > 0.0f;
-    if (metres)
-        for (uint32_t k = 1; k < e->nz; ++k) E_CHECK(e, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
-    bool any = false;
-    for (char r : e->met_ready) any = any || r;
-    if (any) E_CHECK(e, std::equal(lh, lh + e->nz, e->lh_host.begin()), "members must share the level heights");
-    e->lh_host.assign(lh, lh + e->nz);
-    e->has_heights = metres;
-    e->zmax = metres ? lh[e->nz - 1] : (float)(e->nz - 1);
-    E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->nz * 4, hipMemcpyHostToDevice, e->stream));
-    E_HIP(e, fp::launch_pack_met(u0, v0, w0, u1, v1, w1, e->met + (uint64_t)m * e->npts() * 2,
-                                 (uint32_t)e->npts(), e->stream));
-    E_HIP(e, hipStreamSynchronize(e->stream));
-    e->met_ready[m] = 1;
-    return FP_OK;
+    const float *f[6] = {u0, v0, w0, u1, v1, w1};
+    return met_member_common(e, m, f, hipMemcpyDeviceToDevice, lh);
 }
 
 int fp_ens_pbl_upload(fp_ens *e, const float *ustar, const float *wstar, const float *hmix,

@@ -127,17 +127,23 @@ def point_release(n_particles: int = 100_000, seed: int = 42) -> Workload:
 
 
 def _fourier_field(rng, nx, ny, kmax, rms):
+    """sum over 0 <= kx, ky <= kmax, (kx, ky) != 0 of a cos(kx lon + 2 ky lat + ph),
+    a ~ N(0, 1), ph ~ U[0, 2 pi) drawn in (kx, ky) order, scaled to the given rms.
+    Evaluated as Re(Ex^T C Ey) with Ex = exp(i kx lon), Ey = exp(2 i ky lat)."""
     lon = np.deg2rad(np.arange(nx, dtype=np.float64))
     lat = np.deg2rad(np.arange(ny, dtype=np.float64) - 90.0)
-    L, P = np.meshgrid(lon, lat, indexing="ij")
-    f = np.zeros((nx, ny))
+    k = np.arange(kmax + 1)
+    coef = np.zeros((kmax + 1, kmax + 1), np.complex128)
     for kx in range(0, kmax + 1):
         for ky in range(0, kmax + 1):
             if kx == 0 and ky == 0:
                 continue
             a = rng.normal()
             ph = rng.uniform(0, 2 * np.pi)
-            f += a * np.cos(kx * L + 2 * ky * P + ph)
+            coef[kx, ky] = a * np.exp(1j * ph)
+    ex = np.exp(1j * np.outer(lon, k))        # (nx, K)
+    ey = np.exp(2j * np.outer(lat, k))        # (ny, K)
+    f = np.real(ex @ coef @ ey.T)
     f *= rms / max(np.sqrt(np.mean(f ** 2)), 1e-12)
     return f
 
@@ -246,3 +252,42 @@ def small_random(n_particles: int, shape=(64, 64, 32), seed: int = 7, n_levels: 
     return Workload("small_random", met, pbl, parts, 600.0, 4, scale, scale, (0xDECAFBAD, 0x12345678),
                     (shape[0] * 2, shape[1] * 2, 4), (100.0, 500.0, 1000.0, 3000.0), (2, 2), 1.0e6,
                     dry_vd=2.0e-3)
+
+
+@dataclass
+class EnsembleSpec:
+    """Batched-ensemble workload: member sizes, release steps and Philox keys."""
+    counts: np.ndarray        # particles per member
+    start_step: np.ndarray    # sync step of each member's release
+    keys: list                # (key0, key1) per member
+    seed: int
+    dt_seconds: float = 600.0
+
+
+def ensemble_spec(n_members: int = 96, median: float = 200_000.0, sigma: float = 1.0,
+                  lo: int = 10_000, hi: int = 4_000_000, release_hours: float = 6.0,
+                  dt_seconds: float = 600.0, seed: int = 3) -> EnsembleSpec:
+    """96 members with lognormal particle counts (median 200k, sigma 1,
+    clipped to [10k, 4M]; about 30M in total) released at U[0, 6 h)."""
+    rng = np.random.default_rng(seed * 15485863)
+    counts = np.clip(np.round(median * np.exp(sigma * rng.normal(size=n_members))), lo, hi)
+    start = rng.uniform(0.0, release_hours * 3600.0, n_members)
+    keys = [((seed * 1000 + m) & 0xFFFFFFFF, 0x5EED0000 ^ m) for m in range(n_members)]
+    return EnsembleSpec(counts.astype(np.int64), np.floor(start / dt_seconds).astype(np.int64), keys,
+                        seed, dt_seconds)
+
+
+def ensemble_member_particles(spec: EnsembleSpec, m: int) -> np.ndarray:
+    """Member m's particles: 4 point sources of its own (global_sources)."""
+    n = int(spec.counts[m])
+    x, y, z, mass = global_sources(n, seed=spec.seed * 100 + m, n_sources=4)
+    return make_particles(n, x, y, z, mass)
+
+
+def ensemble_member_factor(spec: EnsembleSpec, m: int, nx: int = 360, ny: int = 181,
+                           amplitude: float = 0.1) -> np.ndarray:
+    """Member m's wind perturbation as a multiplicative (nx, ny) factor
+    1 + amplitude * F, F a member-seeded unit-rms Fourier field (wavenumbers
+    <= 8) applied to u, v and w at every level of the base met."""
+    rng = np.random.default_rng(spec.seed * 7 + 1000 * (m + 1))
+    return (1.0 + amplitude * _fourier_field(rng, nx, ny, 8, 1.0)).astype(np.float32)

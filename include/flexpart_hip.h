@@ -298,8 +298,11 @@ int fp_ens_sync(fp_ens *ens);
 /* member slots [0, n) from AoS records; n <= that member's capacity */
 int fp_ens_upload_member(fp_ens *ens, uint32_t member, const fp_particle *p, uint64_t n);
 int fp_ens_download_member(fp_ens *ens, uint32_t member, fp_particle *out, uint64_t n);
-/* member met bracket from device pointers (layout of fp_met_upload_bracket);
- * the level heights must be the same for every member */
+/* member met bracket (layout of fp_met_upload_bracket), from host or device
+ * pointers; the level heights (host) must be the same for every member */
+int fp_ens_met_upload_member(fp_ens *ens, uint32_t member, const float *u0, const float *v0,
+                             const float *w0, const float *u1, const float *v1, const float *w1,
+                             const float *level_heights);
 int fp_ens_met_upload_member_dev(fp_ens *ens, uint32_t member, const float *u0, const float *v0,
                                  const float *w0, const float *u1, const float *v1,
                                  const float *w1, const float *level_heights_host);

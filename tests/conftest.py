@@ -14,6 +14,16 @@ import sys
 import numpy as np
 import pytest
 
+# PyTorch ships its own HIP runtime with the same soname as /opt/rocm's.  The
+# first one loaded serves every later library in the process, and torch cannot
+# start its GPU side on top of the other one, so torch is imported before the
+# library is loaded whenever it is installed (tests hand torch device buffers
+# to the library).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG_DIR = os.path.join(ROOT, "flexpart-gpu_amd")
 sys.path.insert(0, os.path.join(ROOT, "oracle"))

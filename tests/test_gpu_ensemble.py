@@ -60,18 +60,24 @@ CASES = {
 }
 
 
-def build_ensemble(fp, torch, w, keep):
+def build_ensemble(fp, w, device_met=False):
+    """Ensemble of the CAPS members; device_met uploads the met brackets from
+    torch device tensors (fp_ens_met_upload_member_dev) instead of host arrays."""
     ens = fp.GpuEnsemble(CAPS, w.met.shape, n_species=4, out_shape=w.out_shape)
     for m in range(len(CAPS)):
         lv = member_levels(w, m)
-        ts = [torch.from_numpy(a).cuda() for b in lv for a in b]
-        keep.extend(ts)
-        ens.upload_met_member_dev(m, [t.data_ptr() for t in ts], w.met.level_heights)
+        if device_met:
+            import torch
+            ts = [torch.from_numpy(a).cuda() for b in lv for a in b]
+            torch.cuda.synchronize()
+            ens.upload_met_member_dev(m, [t.data_ptr() for t in ts], w.met.level_heights)
+            del ts
+        else:
+            ens.upload_met_member(m, *lv[0], *lv[1], w.met.level_heights)
         ens.upload_member(m, member_particles(w, m))
     ens.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli, w.pbl.precip)
     ens.configure_grid(w.outheights, w.refine, 0, w.mass_scale)
     ens.clear_grid()
-    torch.cuda.synchronize()
     return ens
 
 
@@ -108,14 +114,14 @@ def oracle_member(orc, w, m, cfg, n_steps):
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_ensemble_members_match_oracle_bitwise(fp, orc, small, case, use_graph, launch):
-    import torch
     from test_gpu_parity import base_cfg
     say(f"ensemble parity: case={case} graph={use_graph} launch={launch} members={len(CAPS)}")
     n_steps = 3
     want = [oracle_member(orc, small, m, base_cfg(fp, small, **CASES[case]), n_steps)
             for m in range(len(CAPS))]
-    keep = []
-    with build_ensemble(fp, torch, small, keep) as ens:
+    # the device-pointer met path on one case, the host path on the others
+    device_met = case == "production" and launch == (256, 1)
+    with build_ensemble(fp, small, device_met) as ens:
         ens.set_launch(*launch)
         cfg = base_cfg(fp, small, **CASES[case])
         keys = [member_key(small, m) for m in range(len(CAPS))]
@@ -133,17 +139,14 @@ def test_ensemble_members_match_oracle_bitwise(fp, orc, small, case, use_graph, 
                 counts, mass = ens.download_grid_member(m)
                 assert np.array_equal(counts, want[m][2][0]), f"member {m} grid counts"
                 assert np.array_equal(mass, want[m][2][1]), f"member {m} grid mass"
-    del keep
 
 
 def test_graph_replays_many_steps_equal_direct_launches(fp, small):
-    import torch
     from test_gpu_parity import base_cfg
     say("ensemble: 12 graph replays vs 12 direct launches")
     outs = []
     for use_graph in (False, True):
-        keep = []
-        with build_ensemble(fp, torch, small, keep) as ens:
+        with build_ensemble(fp, small) as ens:
             cfg = base_cfg(fp, small, **CASES["production"])
             ens.configure(cfg, keys=[member_key(small, m) for m in range(len(CAPS))],
                           counters=[member_counter(m) for m in range(len(CAPS))], dispatch=DISPATCH,
@@ -152,7 +155,6 @@ def test_graph_replays_many_steps_equal_direct_launches(fp, small):
             ctr = ens.step(7, use_graph=use_graph)
             outs.append(([ens.download_member(m) for m in range(len(CAPS))], ctr,
                          [ens.download_grid_member(m) for m in range(len(CAPS))]))
-        del keep
     (pa, ca, ga), (pb, cb, gb) = outs
     assert np.array_equal(ca, cb)
     for m in range(len(CAPS)):
@@ -163,12 +165,10 @@ def test_graph_replays_many_steps_equal_direct_launches(fp, small):
 def test_member_equals_single_run_context(fp, small):
     """A member advanced in an ensemble equals the same member stepped alone
     by fp_step with the clock's alpha."""
-    import torch
     from test_gpu_parity import base_cfg
     say("ensemble member vs fp_step")
     m = 3
-    keep = []
-    with build_ensemble(fp, torch, small, keep) as ens:
+    with build_ensemble(fp, small) as ens:
         cfg = base_cfg(fp, small, **CASES["production"])
         ens.configure(cfg, keys=[member_key(small, j) for j in range(len(CAPS))],
                       counters=[member_counter(j) for j in range(len(CAPS))], dispatch=DISPATCH,
@@ -189,12 +189,10 @@ def test_member_equals_single_run_context(fp, small):
             c.key, c.counter = member_key(small, m), counter
             counter = ctx.step(c)
         alone = ctx.download_particles()
-    del keep
     assert_particles_equal(got, alone, "ensemble member vs fp_step")
 
 
 def test_ensemble_errors_are_reported(fp, small):
-    import torch
     from test_gpu_parity import base_cfg
     with pytest.raises(fp.FpError) as e:
         fp.GpuEnsemble([], small.met.shape)
@@ -213,11 +211,10 @@ def test_ensemble_errors_are_reported(fp, small):
             ens.upload_member(0, small.particles[:11])
         assert e.value.code == fp.FP_E_INVALID
         lv = member_levels(small, 0)
-        ts = [torch.from_numpy(a).cuda() for b in lv for a in b]
-        ens.upload_met_member_dev(0, [t.data_ptr() for t in ts], small.met.level_heights)
+        ens.upload_met_member(0, *lv[0], *lv[1], small.met.level_heights)
         with pytest.raises(fp.FpError):
-            ens.upload_met_member_dev(1, [t.data_ptr() for t in ts], small.met.level_heights * 1.5)
-        ens.upload_met_member_dev(1, [t.data_ptr() for t in ts], small.met.level_heights)
+            ens.upload_met_member(1, *lv[0], *lv[1], small.met.level_heights * 1.5)
+        ens.upload_met_member(1, *lv[0], *lv[1], small.met.level_heights)
         ens.upload_pbl(small.pbl.ustar, small.pbl.wstar, small.pbl.hmix, small.pbl.oli)
         with pytest.raises(fp.FpError) as e:
             ens.configure(base_cfg(fp, small, wet_mode=3, wet_a=1e-4, wet_b=0.8, wet_fraction=1.0))
@@ -233,4 +230,3 @@ def test_ensemble_errors_are_reported(fp, small):
         ens.step(2)
         with pytest.raises(fp.FpError):
             ens.set_launch(100, 1)
-        del ts
