@@ -240,11 +240,12 @@ def bench_ensemble(args) -> int:
     """96 members (lognormal sizes, median 200k, about 32M particles) released
     over the first 6 h, each with its own 10 % wind perturbation of the global
     360 x 181 x 60 met, its own Philox key and its own 360 x 181 x 5 grid.
-    After the releases (untimed) and an untimed launch-shape sweep, the same
-    36-step window (6 h, 600 s steps) is timed three ways: a loop over
-    per-member contexts (fp_step per member, each on its own stream), the
-    ensemble with one direct launch per step, and the ensemble replaying its
-    captured graph."""
+    After the releases (untimed) and an untimed launch-shape sweep, a 36-step
+    window (6 h, 600 s steps) is timed four ways: a loop over per-member
+    contexts issuing every member's fp_step each step with no host wait (each
+    context on its own stream), the plain member-after-member loop with a
+    stream synchronisation after each step, the ensemble with one direct
+    launch per step, and the ensemble replaying its captured graph."""
     import ctypes as C
 
     import numpy as np
@@ -363,6 +364,18 @@ def bench_ensemble(args) -> int:
         c.sync()
     t_loop = time.perf_counter() - t0
 
+    # (a') the plain loop as a per-member driver runs it: member after member,
+    # each step followed by a stream synchronisation (the point where a
+    # driver reads back or inspects its state)
+    t0 = time.perf_counter()
+    for h, p, nxt in params:
+        for k in range(args.steps):
+            p.alpha = alpha[min(k, window)]
+            if lib.fp_step(h, C.byref(p), nxt) != 0 or lib.fp_sync(h) != 0:
+                raise RuntimeError(lib.fp_last_error(h).decode())
+            C.memmove(p.counter, nxt, 16)
+    t_serial = time.perf_counter() - t0
+
     # (b) ensemble, one direct launch per step; (c) the captured graph
     ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
     ens.sync()
@@ -400,11 +413,13 @@ def bench_ensemble(args) -> int:
             "particles": total,
             "block": block,
             "particles_per_thread": ppt,
-            "loop_particle_steps_per_s": ps / t_loop,
+            "loop_streams_particle_steps_per_s": ps / t_loop,
+            "loop_serial_particle_steps_per_s": ps / t_serial,
             "direct_particle_steps_per_s": ps / t_direct,
             "graph_particle_steps_per_s": ps / t_graph,
-            "speedup_graph_vs_loop": t_loop / t_graph,
-            "speedup_direct_vs_loop": t_loop / t_direct,
+            "speedup_graph_vs_loop_serial": t_serial / t_graph,
+            "speedup_graph_vs_loop_streams": t_loop / t_graph,
+            "speedup_direct_vs_loop_streams": t_loop / t_direct,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},

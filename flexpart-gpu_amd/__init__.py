@@ -154,6 +154,8 @@ ABI = {
     "fp_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "fp_autotune": (C.c_int, [_vp, C.POINTER(StepParams), C.POINTER(AutotuneParams),
                               C.POINTER(AutotuneResult)]),
+    "fp_interpolation_alpha": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int,
+                                         C.POINTER(C.c_float)]),
     "fp_ens_create": (C.c_int, [C.c_int, C.POINTER(EnsDesc), C.POINTER(_vp)]),
     "fp_ens_destroy": (C.c_int, [_vp]),
     "fp_ens_last_error": (C.c_char_p, [_vp]),
@@ -450,6 +452,19 @@ class GpuTimestep:
                 "step_ms": r.step_ms, "step_ms_with_sort": r.step_ms_with_sort}
 
 
+def interpolation_alpha(t0_s: int, t1_s: int, target_s: int, strict: bool = False) -> float:
+    """Temporal weight of the later met level (fp_interpolation_alpha;
+    simulation/timeloop.rs:2086-2120).  Raises FpError outside the bracket
+    when strict, and for an empty bracket."""
+    lib = library()
+    a = C.c_float()
+    rc = lib.fp_interpolation_alpha(int(t0_s), int(t1_s), int(target_s), 1 if strict else 0,
+                                    C.byref(a))
+    if rc != FP_OK:
+        raise FpError(rc, lib.fp_last_error(None).decode())
+    return float(a.value)
+
+
 def _grid_params(outheights, refine, species, mass_scale) -> GridParams:
     g = GridParams()
     g.refine_x, g.refine_y = refine
@@ -589,6 +604,6 @@ class GpuEnsemble:
 
 
 __all__ = [
-    "GpuTimestep", "GpuEnsemble", "StepConfig", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
+    "GpuTimestep", "GpuEnsemble", "StepConfig", "interpolation_alpha", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
     "library", "ABI", "LIB_PATH",
 ]

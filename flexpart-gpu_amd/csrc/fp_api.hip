@@ -649,6 +649,19 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
     return FP_OK;
 }
 
+int fp_interpolation_alpha(int64_t t0_s, int64_t t1_s, int64_t target_s, int strict, float *alpha)
+{
+    if (!alpha) return fail(nullptr, FP_E_INVALID, "null argument");
+    if (t1_s <= t0_s) return fail(nullptr, FP_E_INVALID, "invalid time bracket: t1 <= t0");
+    int64_t t = target_s;
+    if (t < t0_s || t > t1_s) {
+        if (strict) return fail(nullptr, FP_E_INVALID, "target time outside the met bracket");
+        t = std::min(std::max(t, t0_s), t1_s);
+    }
+    *alpha = (float)((double)(t - t0_s) / (double)(t1_s - t0_s));
+    return FP_OK;
+}
+
 int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n)
 {
     FP_CHECK(ctx, ctx && x && y && (out || n == 0), "null argument");

@@ -11,6 +11,8 @@
 #include "fp_internal.hpp"
 #include "fp_math.h"
 
+static_assert(sizeof(fp_ens_desc) == 40, "fp_ens_desc layout is part of the ABI");
+
 struct fp_ens {
     int device = 0;
     hipStream_t stream = nullptr;

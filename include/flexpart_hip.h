@@ -202,6 +202,18 @@ int fp_compact(fp_ctx *ctx, uint64_t *active_count, uint32_t *survivor_map);
  * when new_to_old is given. */
 int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *params, uint32_t *new_to_old);
 
+/* ---- temporal weight ------------------------------------------------ */
+
+/* interpolation_alpha (simulation/timeloop.rs:2086-2120): the weight of the
+ * later met level at target_s inside the bracket [t0_s, t1_s] (integer
+ * seconds), (target - t0) / (t1 - t0) in binary64 rounded once to float.
+ * A target outside the bracket is clamped (strict = 0) or rejected with
+ * FP_E_INVALID (strict = 1); t1_s <= t0_s is FP_E_INVALID.  Host only. */
+#define FP_TIME_CLAMP 0
+#define FP_TIME_STRICT 1
+int fp_interpolation_alpha(int64_t t0_s, int64_t t1_s, int64_t target_s, int strict,
+                           float *alpha);
+
 /* ---- RNG known-answer path ------------------------------------------ */
 
 /* philox_rng.wgsl (gpu/rng.rs): n consecutive Philox4x32-10 blocks at

@@ -81,7 +81,7 @@ def build_ensemble(fp, w, device_met=False):
     return ens
 
 
-def oracle_member(orc, w, m, cfg, n_steps):
+def oracle_member(fp, orc, w, m, cfg, n_steps):
     """Oracle states after each step, counters after each step and the
     member's fused grid sum over the steps (u32 wrapping, as on the device)."""
     from test_gpu_parity import oracle_params
@@ -96,7 +96,8 @@ def oracle_member(orc, w, m, cfg, n_steps):
     gc = np.zeros(ncell, np.uint32)
     gm = np.zeros(ncell, np.uint32)
     for k in range(n_steps):
-        alpha = float(np.float32((T0 + k * float(dt)) / BRACKET))
+        # the clock's weight is interpolation_alpha at the step's start time
+        alpha = fp.interpolation_alpha(0, int(BRACKET), int(T0 + k * dt))
         cfg.alpha, cfg.counter, cfg.key = alpha, counter, member_key(w, m)
         nxt = orc.step(parts, met, pbl_dict(w.pbl), oracle_params(w, cfg), n_dispatch=DISPATCH[m])
         counter = tuple(int(v) for v in nxt)
@@ -117,7 +118,7 @@ def test_ensemble_members_match_oracle_bitwise(fp, orc, small, case, use_graph, 
     from test_gpu_parity import base_cfg
     say(f"ensemble parity: case={case} graph={use_graph} launch={launch} members={len(CAPS)}")
     n_steps = 3
-    want = [oracle_member(orc, small, m, base_cfg(fp, small, **CASES[case]), n_steps)
+    want = [oracle_member(fp, orc, small, m, base_cfg(fp, small, **CASES[case]), n_steps)
             for m in range(len(CAPS))]
     # the device-pointer met path on one case, the host path on the others
     device_met = case == "production" and launch == (256, 1)
@@ -185,7 +186,7 @@ def test_member_equals_single_run_context(fp, small):
         for k in range(4):
             c = base_cfg(fp, small, **CASES["production"])
             c.grid_fused = False
-            c.alpha = float(np.float32((T0 + k * small.dt_seconds) / BRACKET))
+            c.alpha = fp.interpolation_alpha(0, int(BRACKET), int(T0 + k * small.dt_seconds))
             c.key, c.counter = member_key(small, m), counter
             counter = ctx.step(c)
         alone = ctx.download_particles()
