@@ -94,8 +94,12 @@ def main(argv=None) -> int:
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
     ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("global_10m", "ensemble"), default="global_10m",
-                    help="global_10m (the headline line) or the 96-member batched ensemble")
+    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint"), default="global_10m",
+                    help="global_10m (the headline line), the 96-member batched ensemble, or the "
+                         "differentiable 2M-particle run (adjoint/forward time and FD checks)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size "
+                         "kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -108,6 +112,9 @@ def main(argv=None) -> int:
     if args.workload == "ensemble":
         args.steps = 36 if args.steps is None else args.steps
         return bench_ensemble(args)
+    if args.workload == "adjoint":
+        args.steps = 72 if args.steps is None else args.steps
+        return bench_adjoint(args)
     args.steps = 144 if args.steps is None else args.steps
 
     import numpy as np
@@ -122,6 +129,13 @@ def main(argv=None) -> int:
     print(f"# workload generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda:0")
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     levels = [[torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in lvl] for lvl in w.met.levels]
     torch.cuda.synchronize()
 
@@ -135,7 +149,11 @@ def main(argv=None) -> int:
         ctx.clear_grid()
 
     def step_cfg(k, counter):
-        return fp.StepConfig(dt_seconds=w.dt_seconds, alpha=w.alpha(k * w.dt_seconds)[1],
+        t = int(k * w.dt_seconds)
+        b = w.alpha(t)[0]
+        dtm = int(w.met.seconds_between_levels)
+        alpha = fp.interpolation_alpha(b * dtm, (b + 1) * dtm, t)  # timeloop.rs:2086-2120
+        return fp.StepConfig(dt_seconds=w.dt_seconds, alpha=alpha,
                              x_scale=w.x_scale, y_scale=w.y_scale, key=w.key, counter=counter,
                              dry_mode=1, dry_vd=2e-3, grid_fused=True)
 
@@ -157,15 +175,18 @@ def main(argv=None) -> int:
         return counter
 
     # autotune launch shape and re-sort interval (untimed) on the state half
-    # way through the run, where the plumes have spread
-    reset()
-    counter = run(args.tune_at, 0)
-    tuned = ctx.autotune(step_cfg(args.tune_at, counter))
+    # way through the run, where the plumes have spread; skipped when all
+    # three are given
+    tuned = None
+    if not (args.block and args.ppt and args.sort_every >= 0):
+        reset()
+        counter = run(args.tune_at, 0)
+        tuned = ctx.autotune(step_cfg(args.tune_at, counter))
+        print(f"# autotune at step {args.tune_at}: {tuned}", file=sys.stderr, flush=True)
     block = args.block or tuned["block_size"]
     ppt = args.ppt or tuned["particles_per_thread"]
     ctx.set_launch(block, ppt)
     sort_every = tuned["sort_interval"] if args.sort_every < 0 else args.sort_every
-    print(f"# autotune at step {args.tune_at}: {tuned}", file=sys.stderr, flush=True)
 
     reset()
     run(args.warmup, sort_every)
@@ -428,6 +449,120 @@ def bench_ensemble(args) -> int:
     for c in loop:
         c.close()
     ens.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """The differentiable run: 2M particles from 1024 sources (lognormal
+    emissions, uniform-random positions, heights U[10, 500] m), 12 h (72 steps
+    of 600 s) on the 360 x 181 x 60 global met with 3 time levels, 64 wind
+    coefficients, 2000 receptors on lattice nodes the plumes reach (met columns
+    x 11 levels 500 m apart), observed from a forward run with perturbed-truth
+    parameters over the last 2 h; checkpoints every 8 steps.  Times forward and
+    backward (HIP events on the library stream; median of 3 after a warm-up
+    pair) and checks gradients against central differences with relative step
+    1e-3."""
+    import numpy as np
+    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    n_steps = args.steps
+    c = W.adjoint_case()
+    nx, ny, nz = c.met.shape
+    obs_nz, obs_dz = 11, 500.0
+    obs_first = max(1, n_steps - 11)
+    adj = fp.GpuAdjoint(c.counts, c.met.shape, n_met_levels=len(c.met.levels), n_modes=len(c.coeff),
+                        obs_nz=obs_nz, obs_dz_m=obs_dz, n_steps=n_steps, obs_first_step=obs_first,
+                        checkpoint_every=8)
+    adj.upload_met(c.met.levels, c.met.level_heights, c.met.seconds_between_levels)
+    adj.upload_basis(c.basis_u, c.basis_v)
+    adj.upload_pbl(c.pbl.ustar, c.pbl.wstar, c.pbl.hmix, c.pbl.oli)
+    cfg = fp.StepConfig(dt_seconds=c.dt_seconds, x_scale=c.x_scale, y_scale=c.y_scale, key=c.key,
+                        dry_mode=1, dry_vd=2e-3)
+    # receptors: 2000 random lattice nodes reached by the perturbed-truth run
+    rng = np.random.default_rng(17)
+    q_true = (c.emission * np.exp(0.3 * rng.normal(size=len(c.emission)))).astype(np.float32)
+    r_true = c.position.copy()
+    r_true[:, :2] += rng.normal(0.0, 0.3, (len(c.emission), 2)).astype(np.float32)
+    c_true = rng.normal(0.0, 0.5, len(c.coeff)).astype(np.float32)
+    i, j, l = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(obs_nz), indexing="ij")
+    nodes = np.stack([i.ravel(), j.ravel(), l.ravel()], axis=1)
+    adj.set_receptors(nodes)
+    truth, _ = adj.forward(cfg, q_true, r_true, c_true)
+    hit = np.nonzero(truth > 0)[0]
+    pick = np.sort(rng.choice(hit, size=min(2000, len(hit)), replace=False))
+    adj.set_receptors(nodes[pick], truth[pick])
+    print(f"# adjoint: {len(c.counts)} sources, {adj.n_particles} particles, {n_steps} steps, "
+          f"{len(pick)} receptors of {len(hit)} reached nodes", file=sys.stderr, flush=True)
+
+    adj.forward(cfg, c.emission, c.position, c.coeff)
+    adj.backward()
+    fwd, bwd = [], []
+    for _ in range(3):
+        _, loss = adj.forward(cfg, c.emission, c.position, c.coeff)
+        gq, gr, gc = adj.backward()
+        f, b = adj.last_ms()
+        fwd.append(f)
+        bwd.append(b)
+    fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
+
+    def fd(which, idx, h):
+        vals = []
+        for sgn in (1.0, -1.0):
+            q, r, k = c.emission.copy(), c.position.copy(), c.coeff.copy()
+            arr = {"q": q, "r": r, "c": k}[which]
+            arr[idx] = np.float32(arr[idx] + sgn * h)
+            vals.append(adj.forward(cfg, q, r, k)[1])
+        base = {"q": c.emission, "r": c.position, "c": c.coeff}[which][idx]
+        return (vals[0] - vals[1]) / float(np.float32(base + h) - np.float32(base - h))
+
+    checks = {}
+    for name, g, which, step_of in (
+            ("emission", gq, "q", lambda v: 1e-3 * abs(float(v))),
+            ("position_xy", gr[:, :2], "r", lambda v: 1e-3),
+            ("wind", gc, "c", lambda v: 1e-3)):
+        flat = np.argsort(np.abs(g).ravel())[-3:]
+        rows = []
+        for f in flat:
+            idx = np.unravel_index(f, g.shape)
+            full_idx = idx if which != "r" else (idx[0], idx[1])
+            val = {"q": c.emission, "r": c.position, "c": c.coeff}[which][full_idx]
+            d = fd(which, full_idx, step_of(val))
+            rows.append({"index": [int(x) for x in full_idx], "adjoint": float(g[idx]), "fd": d,
+                         "rel_err": abs(float(g[idx]) - d) / max(abs(d), 1e-30)})
+        checks[name] = rows
+    adj.close()
+    particle_steps = adj.n_particles * n_steps
+    result = {
+        "metric": "adjoint/forward time (2M particles, 72 steps, 1 GPU)",
+        "value": bwd_ms / fwd_ms,
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": n_steps,
+        "warmup": 1,
+        "ms_per_step": fwd_ms / n_steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "adjoint_2m: 2M particles from 1024 point sources (lognormal emissions), "
+                        "360x181x60 met with 3 levels 6 h apart, 72 steps of 600 s, 64 wind "
+                        "coefficients, 2000 receptors (last 2 h), checkpoints every 8 steps",
+            "particles": adj.n_particles,
+            "forward_ms": fwd_ms,
+            "backward_ms": bwd_ms,
+            "forward_particle_steps_per_s": particle_steps / (fwd_ms / 1e3),
+            "loss": loss,
+            "fd_checks": checks,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
     print(json.dumps(result), flush=True)
     return 0
 

@@ -117,7 +117,16 @@ class EnsDesc(C.Structure):
                 ("capacities", C.POINTER(C.c_uint64))]
 
 
+class AdjDesc(C.Structure):
+    _fields_ = [("n_sources", C.c_uint32), ("met_nx", C.c_uint32), ("met_ny", C.c_uint32),
+                ("met_nz", C.c_uint32), ("n_met_levels", C.c_uint32), ("n_modes", C.c_uint32),
+                ("obs_nz", C.c_uint32), ("obs_dz_m", C.c_float), ("n_steps", C.c_uint32),
+                ("obs_first_step", C.c_uint32), ("checkpoint_every", C.c_uint32),
+                ("reserved", C.c_uint32), ("particles_per_source", C.POINTER(C.c_uint32))]
+
+
 _f = C.POINTER(C.c_float)
+_d = C.POINTER(C.c_double)
 _u32 = C.POINTER(C.c_uint32)
 _u64 = C.POINTER(C.c_uint64)
 _vp = C.c_void_p
@@ -140,6 +149,7 @@ ABI = {
     "fp_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
     "fp_forcing_upload": (C.c_int, [_vp, _f, _f, _f]),
     "fp_step": (C.c_int, [_vp, C.POINTER(StepParams), _u32]),
+    "fp_step_vjp": (C.c_int, [_vp, C.POINTER(StepParams), _f, _f, _d, _d]),
     "fp_grid_clear": (C.c_int, [_vp]),
     "fp_grid_configure": (C.c_int, [_vp, C.POINTER(GridParams)]),
     "fp_grid_accumulate": (C.c_int, [_vp]),
@@ -156,6 +166,17 @@ ABI = {
                               C.POINTER(AutotuneResult)]),
     "fp_interpolation_alpha": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                          C.POINTER(C.c_float)]),
+    "fp_adj_create": (C.c_int, [C.c_int, C.POINTER(AdjDesc), C.POINTER(_vp)]),
+    "fp_adj_destroy": (C.c_int, [_vp]),
+    "fp_adj_last_error": (C.c_char_p, [_vp]),
+    "fp_adj_met_upload": (C.c_int, [_vp, _f, _f, _f, _f, C.c_double]),
+    "fp_adj_basis_upload": (C.c_int, [_vp, _f, _f]),
+    "fp_adj_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
+    "fp_adj_receptors": (C.c_int, [_vp, C.c_uint32, _u32, _d]),
+    "fp_adj_forward": (C.c_int, [_vp, C.POINTER(StepParams), _f, _f, _f, _d, _d]),
+    "fp_adj_backward": (C.c_int, [_vp, _d, _d, _d]),
+    "fp_adj_download_state": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fp_adj_last_ms": (C.c_int, [_vp, _d, _d]),
     "fp_ens_create": (C.c_int, [C.c_int, C.POINTER(EnsDesc), C.POINTER(_vp)]),
     "fp_ens_destroy": (C.c_int, [_vp]),
     "fp_ens_last_error": (C.c_char_p, [_vp]),
@@ -355,6 +376,22 @@ class GpuTimestep:
         nxt = (C.c_uint32 * 4)()
         self._chk(self._lib.fp_step(self._h, C.byref(p), nxt))
         return tuple(int(v) for v in nxt)
+
+    def step_vjp(self, cfg: StepConfig, lam_out, want_wind: bool = False):
+        """J^T lam_out of one step at the current particles (fp_step_vjp).
+        lam_out: (7, dispatch) for x, y, z, turb_u, turb_v, turb_w, mass[0].
+        Returns lam_in (7, dispatch) and, if asked, the (u, v) wind adjoints
+        per met column."""
+        n = self.particle_count()
+        lo = np.ascontiguousarray(np.asarray(lam_out, np.float32).reshape(7, n))
+        li = np.zeros((7, n), np.float32)
+        nxy = self.met_shape[0] * self.met_shape[1]
+        wu = np.zeros(nxy, np.float64) if want_wind else None
+        wv = np.zeros(nxy, np.float64) if want_wind else None
+        self._chk(self._lib.fp_step_vjp(self._h, C.byref(cfg.to_c()), _fp(lo), _fp(li),
+                                        None if wu is None else wu.ctypes.data_as(_d),
+                                        None if wv is None else wv.ctypes.data_as(_d)))
+        return (li, wu, wv) if want_wind else li
 
     # -- concentration grid
     def configure_grid(self, outheights=None, refine=(1, 1), species=0, mass_scale=1.0e6):
@@ -603,7 +640,149 @@ class GpuEnsemble:
         return float(ms.value)
 
 
+class GpuAdjoint:
+    """Differentiable run of point sources (fp_adj_*): forward to receptor
+    values and misfit, backward to d(misfit)/d(emissions, release positions,
+    wind coefficients).
+
+    ``particles_per_source`` fixes the sources (source s owns a contiguous
+    slot range); the met holds ``n_met_levels`` time levels; receptors are
+    nodes (i, j, l) of the observation lattice (met columns x ``obs_nz``
+    levels ``obs_dz_m`` apart).
+    """
+
+    def __init__(self, particles_per_source, met_shape, n_met_levels: int = 2, n_modes: int = 0,
+                 obs_nz: int = 11, obs_dz_m: float = 500.0, n_steps: int = 72,
+                 obs_first_step: int = 1, checkpoint_every: int = 8, device: int = 0):
+        lib = library()
+        counts = np.ascontiguousarray(np.asarray(particles_per_source, np.uint32))
+        nx, ny, nz = met_shape
+        d = AdjDesc(len(counts), nx, ny, nz, n_met_levels, n_modes, obs_nz, obs_dz_m, n_steps,
+                    obs_first_step, checkpoint_every, 0, counts.ctypes.data_as(_u32))
+        h = _vp()
+        rc = lib.fp_adj_create(device, C.byref(d), C.byref(h))
+        if rc != FP_OK:
+            raise FpError(rc, lib.fp_adj_last_error(None).decode())
+        self._lib = lib
+        self._h = h
+        self.counts = counts.copy()
+        self.n_sources = len(counts)
+        self.n_particles = int(counts.sum())
+        self.met_shape = (nx, ny, nz)
+        self.n_met_levels = n_met_levels
+        self.n_modes = n_modes
+        self.n_receptors = 0
+
+    def _chk(self, rc):
+        if rc != FP_OK:
+            raise FpError(rc, self._lib.fp_adj_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.fp_adj_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_met(self, levels, level_heights, seconds_between_levels: float):
+        """levels: n_met_levels tuples (u, v, w) of flat met arrays."""
+        if len(levels) != self.n_met_levels:
+            raise ValueError(f"expected {self.n_met_levels} met levels")
+        n = int(np.prod(self.met_shape))
+        u, v, w = (np.ascontiguousarray(np.concatenate([_f32(l[c], n) for l in levels]))
+                   for c in range(3))
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_adj_met_upload(self._h, _fp(u), _fp(v), _fp(w), _fp(lh),
+                                              float(seconds_between_levels)))
+
+    def upload_basis(self, basis_u, basis_v):
+        n = self.n_modes * self.met_shape[0] * self.met_shape[1]
+        self._chk(self._lib.fp_adj_basis_upload(self._h, _fp(_f32(basis_u, n)), _fp(_f32(basis_v, n))))
+
+    def upload_pbl(self, ustar, wstar, hmix, oli, precip=None):
+        n = self.met_shape[0] * self.met_shape[1]
+        arrs = [_f32(a, n) for a in (ustar, wstar, hmix, oli)]
+        pr = None if precip is None else _f32(precip, n)
+        self._chk(self._lib.fp_adj_pbl_upload(self._h, *[_fp(a) for a in arrs], _fp(pr)))
+
+    def set_receptors(self, nodes, observed=None):
+        nd = np.ascontiguousarray(np.asarray(nodes, np.uint32).reshape(-1, 3))
+        ob = None if observed is None else np.ascontiguousarray(np.asarray(observed, np.float64))
+        if ob is not None and ob.size != len(nd):
+            raise ValueError("one observation per receptor")
+        self._chk(self._lib.fp_adj_receptors(self._h, len(nd), nd.ctypes.data_as(_u32),
+                                             None if ob is None else ob.ctypes.data_as(_d)))
+        self.n_receptors = len(nd)
+
+    def forward(self, cfg: StepConfig, emission, position, coeff=None):
+        """Run forward; returns (receptor values, loss)."""
+        q = _f32(emission, self.n_sources)
+        r = _f32(position, 3 * self.n_sources)
+        c = _f32(np.zeros(0) if coeff is None else coeff, self.n_modes)
+        vals = np.zeros(max(self.n_receptors, 1), np.float64)
+        loss = C.c_double()
+        self._chk(self._lib.fp_adj_forward(self._h, C.byref(cfg.to_c()), _fp(q), _fp(r),
+                                           _fp(c) if self.n_modes else None,
+                                           vals.ctypes.data_as(_d), C.byref(loss)))
+        return vals[: self.n_receptors], float(loss.value)
+
+    def backward(self):
+        """Gradient of the last forward's loss: (d/d emission, d/d position (S, 3), d/d coeff)."""
+        gq = np.zeros(self.n_sources, np.float64)
+        gr = np.zeros((self.n_sources, 3), np.float64)
+        gc = np.zeros(max(self.n_modes, 1), np.float64)
+        self._chk(self._lib.fp_adj_backward(self._h, gq.ctypes.data_as(_d), gr.ctypes.data_as(_d),
+                                            gc.ctypes.data_as(_d)))
+        return gq, gr, gc[: self.n_modes]
+
+    def download_state(self, n=None) -> np.ndarray:
+        n = self.n_particles if n is None else int(n)
+        out = np.zeros(n, PARTICLE_DTYPE)
+        self._chk(self._lib.fp_adj_download_state(self._h, out.ctypes.data, n))
+        return out
+
+    def last_ms(self):
+        f, b = C.c_double(), C.c_double()
+        self._chk(self._lib.fp_adj_last_ms(self._h, C.byref(f), C.byref(b)))
+        return float(f.value), float(b.value)
+
+
+def receptor_misfit_function():
+    """torch.autograd.Function over GpuAdjoint: ``apply(adj, cfg, q, r, c)``
+    returns the misfit as a 0-d tensor; backward returns its gradient with
+    respect to q (S), r (S, 3) and c (K).  Host tensors (float64 grads)."""
+    import torch
+
+    class ReceptorMisfit(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, adj, cfg, q, r, c):
+            _, loss = adj.forward(cfg, q.detach().cpu().numpy(), r.detach().cpu().numpy(),
+                                  None if c is None else c.detach().cpu().numpy())
+            ctx.adj = adj
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            gq, gr, gc = ctx.adj.backward()
+            g = float(grad_out)
+            return (None, None, torch.from_numpy(gq * g), torch.from_numpy(gr * g),
+                    torch.from_numpy(gc * g))
+
+    return ReceptorMisfit
+
+
 __all__ = [
-    "GpuTimestep", "GpuEnsemble", "StepConfig", "interpolation_alpha", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
+    "GpuTimestep", "GpuEnsemble", "GpuAdjoint", "receptor_misfit_function", "StepConfig",
+    "interpolation_alpha", "PARTICLE_DTYPE", "SOA_FIELDS", "FpError", "LibraryMissing",
     "library", "ABI", "LIB_PATH",
 ]

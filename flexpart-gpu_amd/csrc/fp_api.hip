@@ -534,7 +534,11 @@ int fp_grid_download(fp_ctx *ctx, uint32_t *counts, uint32_t *mass_fp)
 
 // ---------------------------------------------------------------- step
 
-int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
+}  // extern "C"
+
+// Validated step arguments of fp_step (and fp_step_vjp) on the context's
+// current particle set.
+static int build_step_args(fp_ctx *ctx, const fp_step_params *p, fp::StepArgs &a)
 {
     FP_CHECK(ctx, ctx && p, "null argument");
     if (!ctx->met_ready) return fail(ctx, FP_E_STATE, "fp_step before fp_met_upload_bracket");
@@ -554,7 +558,7 @@ int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
     if (p->grid_fused && !ctx->grid_configured)
         return fail(ctx, FP_E_STATE, "fused gridding before fp_grid_configure");
 
-    fp::StepArgs a{};
+    a = fp::StepArgs{};
     a.s = ctx->s();
     a.n = (uint32_t)ctx->dispatch;
     a.n_species = ctx->desc.n_species;
@@ -573,6 +577,15 @@ int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
     fp::fill_step_scalars(a, *p);
     if (a.grid_fused) a.grid = grid_args(ctx);
     a.ppt = ctx->ppt;
+    return FP_OK;
+}
+
+extern "C" {
+
+int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
+{
+    fp::StepArgs a;
+    if (const int rc = build_step_args(ctx, p, a)) return rc;
 
     fp_ctx::Ev ev{};
     prof_begin(ctx, ev, FP_KERNEL_STEP);
@@ -581,6 +594,42 @@ int fp_step(fp_ctx *ctx, const fp_step_params *p, uint32_t next_counter[4])
 
     if (next_counter)
         fp::counter_advance(p->counter, fp::counter_per_step(ctx->dispatch, p->n_substeps), next_counter);
+    return FP_OK;
+}
+
+int fp_step_vjp(fp_ctx *ctx, const fp_step_params *p, const float *lam_out, float *lam_in, double *wind_u,
+                double *wind_v)
+{
+    fp::StepArgs a;
+    if (const int rc = build_step_args(ctx, p, a)) return rc;
+    FP_CHECK(ctx, lam_out && lam_in, "null argument");
+    FP_CHECK(ctx, (wind_u == nullptr) == (wind_v == nullptr), "wind_u and wind_v go together");
+    a.grid_fused = 0;
+    const uint64_t n = ctx->dispatch;
+    if (n == 0) return FP_OK;
+    const uint64_t nxy = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny;
+    float *lam = nullptr;
+    double *au = nullptr, *av = nullptr;
+    FP_HIP(ctx, dalloc(&lam, n * 7u));
+    auto cleanup = [&]() {
+        if (lam) (void)hipFree(lam);
+        if (au) (void)hipFree(au);
+        if (av) (void)hipFree(av);
+    };
+    hipError_t h = hipMemcpyAsync(lam, lam_out, n * 7u * 4u, hipMemcpyHostToDevice, ctx->stream);
+    if (h == hipSuccess && wind_u) {
+        h = dalloc(&au, nxy);
+        if (h == hipSuccess) h = dalloc(&av, nxy);
+        if (h == hipSuccess) h = hipMemsetAsync(au, 0, nxy * 8u, ctx->stream);
+        if (h == hipSuccess) h = hipMemsetAsync(av, 0, nxy * 8u, ctx->stream);
+    }
+    if (h == hipSuccess) h = fp::launch_adj_step(a, lam, n, au, av, ctx->stream);
+    if (h == hipSuccess) h = hipMemcpyAsync(lam_in, lam, n * 7u * 4u, hipMemcpyDeviceToHost, ctx->stream);
+    if (h == hipSuccess && wind_u) h = hipMemcpyAsync(wind_u, au, nxy * 8u, hipMemcpyDeviceToHost, ctx->stream);
+    if (h == hipSuccess && wind_u) h = hipMemcpyAsync(wind_v, av, nxy * 8u, hipMemcpyDeviceToHost, ctx->stream);
+    if (h == hipSuccess) h = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    FP_HIP(ctx, h);
     return FP_OK;
 }
 

@@ -205,7 +205,7 @@ struct fp_ctx {
 
 namespace fp {
 
-// Kernel launchers (fp_kernels.hip).
+// Kernel launchers (fp_step.hip, fp_aux.hip).
 hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
                            uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st);
@@ -224,6 +224,10 @@ hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2]
                                 uint32_t *out, hipStream_t st);
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
                              hipStream_t st);
+// Adjoint of one step (fp_adj.hip): lam (7 x stride floats: x, y, z, u', v',
+// w', m0 of the step's output) is replaced by the adjoint of its input; au/av
+// (nx*ny doubles, may be null) accumulate the adjoint of the horizontal wind.
+hipError_t launch_adj_step(const StepArgs &a, float *lam, uint64_t stride, double *au, double *av, hipStream_t st);
 // Morton sort: keys + stable radix sort + gather (fp_sort.hip).
 hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n);
 

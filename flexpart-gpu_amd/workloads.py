@@ -291,3 +291,56 @@ def ensemble_member_factor(spec: EnsembleSpec, m: int, nx: int = 360, ny: int = 
     <= 8) applied to u, v and w at every level of the base met."""
     rng = np.random.default_rng(spec.seed * 7 + 1000 * (m + 1))
     return (1.0 + amplitude * _fourier_field(rng, nx, ny, 8, 1.0)).astype(np.float32)
+
+
+@dataclass
+class AdjointCase:
+    """Point sources, wind-perturbation basis and parameters of a
+    differentiable run."""
+    met: Met
+    pbl: Pbl
+    counts: np.ndarray          # particles per source
+    emission: np.ndarray        # kg per source
+    position: np.ndarray        # (S, 3): x, y [grid units], z [m]
+    basis_u: np.ndarray         # (K, nx * ny)
+    basis_v: np.ndarray
+    coeff: np.ndarray           # (K,)
+    dt_seconds: float
+    x_scale: float
+    y_scale: float
+    key: tuple
+
+
+def wind_basis(n_modes: int, nx: int, ny: int, seed: int = 5):
+    """K unit-rms Fourier fields (wavenumbers <= 4): the first half perturbs
+    u, the second half v."""
+    rng = np.random.default_rng(seed * 7907)
+    bu = np.zeros((n_modes, nx * ny), np.float32)
+    bv = np.zeros((n_modes, nx * ny), np.float32)
+    for k in range(n_modes):
+        f = _fourier_field(rng, nx, ny, 4, 1.0).astype(np.float32).reshape(-1)
+        (bu if k < (n_modes + 1) // 2 else bv)[k] = f
+    return bu, bv
+
+
+def adjoint_case(n_particles: int = 2_000_000, n_sources: int = 1024, n_modes: int = 64,
+                 met: Met | None = None, pbl: Pbl | None = None, seed: int = 4,
+                 dt_seconds: float = 600.0, x_scale: float | None = None) -> AdjointCase:
+    """Sources at uniform-random positions (heights U[10, 500] m) with
+    lognormal emissions (median 1 kg, sigma 1), particles split evenly,
+    zero wind coefficients.  Defaults: the 2M-particle, 1024-source case on
+    the 1 deg global met with 3 time levels (12 h)."""
+    met = global_met(2, n_levels=3) if met is None else met
+    pbl = global_pbl(2) if pbl is None else pbl
+    nx, ny, _ = met.shape
+    rng = np.random.default_rng(seed * 6151)
+    counts = np.full(n_sources, n_particles // n_sources, np.uint32)
+    counts[: n_particles % n_sources] += 1
+    margin = min(10.0, (ny - 1) / 4.0)
+    pos = np.stack([rng.uniform(1.0, nx - 2.0, n_sources), rng.uniform(margin, ny - 1.0 - margin, n_sources),
+                    rng.uniform(10.0, 500.0, n_sources)], axis=1).astype(np.float32)
+    q = np.exp(rng.normal(size=n_sources)).astype(np.float32)
+    bu, bv = wind_basis(n_modes, nx, ny, seed)
+    scale = (1.0 / METRES_PER_DEG) if x_scale is None else x_scale
+    return AdjointCase(met, pbl, counts, q, pos, bu, bv, np.zeros(n_modes, np.float32), dt_seconds, scale, scale,
+                       (seed, 0x0AD10AD1))

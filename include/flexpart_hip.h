@@ -176,6 +176,20 @@ int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,
  * (gpu/langevin_fused.rs:229-233). */
 int fp_step(fp_ctx *ctx, const fp_step_params *params, uint32_t next_counter[4]);
 
+/* Vector-Jacobian product of one fp_step at the context's current particles
+ * (the step is not applied): for every dispatched slot, lam_in = J^T lam_out
+ * where J is the derivative of the step's output (x, y [grid units], z [m],
+ * turb_u, turb_v, turb_w, mass[0]) with respect to the same quantities at
+ * its input, under the step's own Philox draws.  lam_out / lam_in: 7 x
+ * dispatch floats (host, quantity-major).  wind_u / wind_v (host, met_nx *
+ * met_ny doubles, both or neither) receive the derivative of sum(lam_out *
+ * output) with respect to a horizontal-wind offset at each met column (added
+ * to every level and time level).  Inactive slots pass lam through.  The
+ * building block of fp_adj_backward; no counterpart in the reference.
+ * Blocking. */
+int fp_step_vjp(fp_ctx *ctx, const fp_step_params *params, const float *lam_out, float *lam_in,
+                double *wind_u, double *wind_v);
+
 /* ---- concentration grid --------------------------------------------- */
 
 /* ConcentrationGridBuffers::clear (gpu/gridding.rs:178-188). */
@@ -341,6 +355,71 @@ int fp_ens_set_launch(fp_ens *ens, uint32_t block_size, uint32_t particles_per_t
 /* mean milliseconds per fp_ens step launch over the last fp_ens_step call
  * (HIP events on the ensemble stream). Blocking. */
 int fp_ens_last_step_ms(fp_ens *ens, double *ms);
+
+/* ---- differentiable runs (adjoint) ----------------------------------
+ * A forward run of point sources through the timestep, and its hand-written
+ * adjoint: the gradient of a receptor misfit
+ *     L = sum_r (c_r - obs_r)^2,
+ *     c_r = sum over observation times t and particles p of m_p(t) W(X_p(t), node_r)
+ * (W: cloud-in-cell weights of the observation lattice: met grid columns in
+ * x and y, obs_nz levels obs_dz_m apart in z) with respect to the emission
+ * of every source (q_s, kg), its release position (x, y in grid units, z in
+ * m) and the coefficients c_k of a wind perturbation
+ *     u += sum_k c_k Bu_k(i, j),  v += sum_k c_k Bv_k(i, j)
+ * added to every level and time level of the met.  Source s releases
+ * particles_per_source[s] particles at its position at t = 0, each with mass
+ * q_s / particles_per_source[s].  The backward pass replays the forward's
+ * Philox draws (counter-based), restarts from a state checkpoint every
+ * checkpoint_every steps, recomputes the steps in between, and propagates
+ * the adjoint state back through deposition, the Langevin sub-steps with
+ * reflection, the Hanna profiles, advection and the wind interpolation.
+ * It is the pathwise (common random numbers) derivative.  No counterpart in
+ * the reference. */
+typedef struct fp_adj fp_adj;
+
+typedef struct fp_adj_desc {
+    uint32_t n_sources;
+    uint32_t met_nx, met_ny, met_nz;
+    uint32_t n_met_levels;            /* time levels, >= 2 */
+    uint32_t n_modes;                 /* wind coefficients */
+    uint32_t obs_nz;                  /* observation lattice levels, >= 2 */
+    float obs_dz_m;                   /* spacing of the lattice levels */
+    uint32_t n_steps;
+    uint32_t obs_first_step;          /* observe after steps obs_first_step..n_steps (>= 1) */
+    uint32_t checkpoint_every;        /* >= 1 */
+    uint32_t reserved;
+    const uint32_t *particles_per_source; /* n_sources counts (>= 1 each) */
+} fp_adj_desc;
+
+int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out);
+int fp_adj_destroy(fp_adj *adj);
+const char *fp_adj_last_error(const fp_adj *adj);
+/* met time levels (host, level-major: u[l*npts + p]; layout of
+ * fp_met_upload_bracket per level) seconds_between_levels apart */
+int fp_adj_met_upload(fp_adj *adj, const float *u, const float *v, const float *w,
+                      const float *level_heights, double seconds_between_levels);
+/* wind perturbation basis: n_modes fields of met_nx * met_ny (host, mode-major) */
+int fp_adj_basis_upload(fp_adj *adj, const float *basis_u, const float *basis_v);
+int fp_adj_pbl_upload(fp_adj *adj, const float *ustar, const float *wstar, const float *hmix,
+                      const float *oli, const float *precip);
+/* receptors as lattice nodes (i, j, l) (host, 3 per receptor) and their
+ * observed values (may be NULL: zero) */
+int fp_adj_receptors(fp_adj *adj, uint32_t n_receptors, const uint32_t *nodes, const double *observed);
+/* Forward run.  base: physics of every step (dt, scales, key, counter at
+ * step 0, sub-steps, deposition; alpha and grid_fused are ignored: the
+ * temporal weight follows interpolation_alpha over the met levels).
+ * emission: n_sources, position: 3 per source, coeff: n_modes.
+ * receptor_values (n_receptors, may be NULL) and loss (may be NULL) on
+ * return.  Blocking. */
+int fp_adj_forward(fp_adj *adj, const fp_step_params *base, const float *emission,
+                   const float *position, const float *coeff, double *receptor_values,
+                   double *loss);
+/* Gradient of the loss of the last forward run.  Blocking. */
+int fp_adj_backward(fp_adj *adj, double *grad_emission, double *grad_position, double *grad_coeff);
+/* particle state after the last forward run (n_particles records) */
+int fp_adj_download_state(fp_adj *adj, fp_particle *out, uint64_t n);
+/* device milliseconds of the last forward and backward calls */
+int fp_adj_last_ms(fp_adj *adj, double *forward_ms, double *backward_ms);
 
 #ifdef __cplusplus
 }

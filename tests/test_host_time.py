@@ -43,6 +43,7 @@ def test_empty_or_reversed_bracket_is_rejected(fp):
         assert e.value.code == fp.FP_E_INVALID
 
 
-def test_ensemble_descriptor_layout(fp):
+def test_ensemble_and_adjoint_descriptor_layouts(fp):
     import ctypes as C
     assert C.sizeof(fp.EnsDesc) == 40
+    assert C.sizeof(fp.AdjDesc) == 56
