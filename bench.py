@@ -94,7 +94,10 @@ def main(argv=None) -> int:
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
     ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint"), default="global_10m",
+    ap.add_argument("--loss-fraction", type=float, default=0.5,
+                    help="stress workload: a particle is lost once its mass falls below this "
+                         "fraction of the common initial particle mass")
+    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress"), default="global_10m",
                     help="global_10m (the headline line), the 96-member batched ensemble, or the "
                          "differentiable 2M-particle run (adjoint/forward time and FD checks)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -115,6 +118,9 @@ def main(argv=None) -> int:
     if args.workload == "adjoint":
         args.steps = 72 if args.steps is None else args.steps
         return bench_adjoint(args)
+    if args.workload == "stress":
+        args.steps = 144 if args.steps is None else args.steps
+        return bench_stress(args)
     args.steps = 144 if args.steps is None else args.steps
 
     import numpy as np
@@ -559,6 +565,113 @@ def bench_adjoint(args) -> int:
             "forward_particle_steps_per_s": particle_steps / (fwd_ms / 1e3),
             "loss": loss,
             "fd_checks": checks,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_stress(args) -> int:
+    """The deposition stress run: 50M particles from 4096 sources on the 0.25
+    deg global met (1440 x 721 x 60, two time levels), 24 h of 600 s steps,
+    dry deposition v_d = 1e-2 m/s, below-cloud scavenging 1e-4 P^0.8 with
+    precipitation in elliptical patches over 20 % of the domain, the loss rule
+    and compaction after every step; receptors observed over the last 2 h.
+    Times the forward (particle-steps/s over the active particles) and the
+    emission adjoint through the saved survivor maps (HIP events on the
+    library stream; median of 2 after a warm-up pair)."""
+    import numpy as np
+    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    n_steps = args.steps
+    t_gen = time.perf_counter()
+    c = W.stress_case()
+    nx, ny, nz = c.met.shape
+    print(f"# stress workload generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    obs_nz, obs_dz = 11, 500.0
+    adj = fp.GpuAdjoint(c.counts, c.met.shape, n_met_levels=2, n_modes=0, obs_nz=obs_nz, obs_dz_m=obs_dz,
+                        n_steps=n_steps, obs_first_step=max(1, n_steps - 11), checkpoint_every=8,
+                        compact=True, emission_only=True)
+    adj.upload_met(c.met.levels, c.met.level_heights, c.met.seconds_between_levels)
+    adj.upload_pbl(c.pbl.ustar, c.pbl.wstar, c.pbl.hmix, c.pbl.oli, c.pbl.precip)
+    m0 = float(np.median(c.emission / c.counts.astype(np.float32)))
+    cfg = fp.StepConfig(dt_seconds=c.dt_seconds, x_scale=c.x_scale, y_scale=c.y_scale, key=c.key,
+                        dry_mode=1, dry_vd=1e-2, wet_mode=3, wet_a=1e-4, wet_b=0.8, wet_fraction=1.0,
+                        loss_mode=1, loss_min_mass=args.loss_fraction * m0)
+    rng = np.random.default_rng(19)
+    q_true = (c.emission * np.exp(0.3 * rng.normal(size=len(c.emission)))).astype(np.float32)
+    i, j, l = np.meshgrid(np.arange(nx, dtype=np.uint32), np.arange(ny, dtype=np.uint32),
+                          np.arange(obs_nz, dtype=np.uint32), indexing="ij")
+    nodes = np.stack([i.ravel(), j.ravel(), l.ravel()], axis=1)
+    del i, j, l
+    adj.set_receptors(nodes)
+    truth, _ = adj.forward(cfg, q_true, c.position)
+    hit = np.nonzero(truth > 0)[0]
+    pick = np.sort(rng.choice(hit, size=min(2000, len(hit)), replace=False))
+    adj.set_receptors(nodes[pick], truth[pick])
+    del nodes, truth
+    adj.forward(cfg, c.emission, c.position)
+    adj.backward()
+    fwd, bwd = [], []
+    for _ in range(2):
+        _, loss = adj.forward(cfg, c.emission, c.position)
+        gq, _, _ = adj.backward()
+        f, b = adj.last_ms()
+        fwd.append(f)
+        bwd.append(b)
+    counts = adj.active_counts().astype(np.float64)
+    fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
+    active_steps = float(np.sum(counts[:-1]))
+    print(f"# stress: {adj.n_particles} particles, {len(pick)} receptors, lost "
+          f"{1 - counts[-1] / counts[0]:.3f}", file=sys.stderr, flush=True)
+
+    def fd(idx):
+        vals = []
+        h = 1e-3 * float(c.emission[idx])
+        for sgn in (1.0, -1.0):
+            q = c.emission.copy()
+            q[idx] = np.float32(q[idx] + sgn * h)
+            vals.append(adj.forward(cfg, q, c.position)[1])
+        return (vals[0] - vals[1]) / float(np.float32(c.emission[idx] + h) - np.float32(c.emission[idx] - h))
+
+    checks = []
+    for idx in np.argsort(np.abs(gq))[-3:]:
+        d = fd(int(idx))
+        checks.append({"source": int(idx), "adjoint": float(gq[idx]), "fd": d,
+                       "rel_err": abs(float(gq[idx]) - d) / max(abs(d), 1e-30)})
+    adj.close()
+    result = {
+        "metric": "particle-steps/s with compaction every step (50M particles, 1 GPU)",
+        "value": active_steps / (fwd_ms / 1e3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": n_steps,
+        "warmup": 1,
+        "ms_per_step": fwd_ms / n_steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "stress_50m: 50M particles from 4096 sources, 1440x721x60 met (0.25 deg), "
+                        "24 h of 600 s steps, dry v_d 1e-2, wet 1e-4 P^0.8 in precip patches over 20 % "
+                        "of the domain, loss rule and compaction every step, emission adjoint",
+            "particles": adj.n_particles,
+            "active_start": int(counts[0]),
+            "active_end": int(counts[-1]),
+            "lost_fraction": 1.0 - counts[-1] / counts[0],
+            "loss_threshold_fraction": args.loss_fraction,
+            "forward_ms": fwd_ms,
+            "emission_adjoint_ms": bwd_ms,
+            "adjoint_over_forward": bwd_ms / fwd_ms,
+            "loss": loss,
+            "fd_checks_emission": checks,
         },
         "roofline": None,
         "cpu_baseline": None,

@@ -117,12 +117,28 @@ class EnsDesc(C.Structure):
                 ("capacities", C.POINTER(C.c_uint64))]
 
 
+class GridDomain(C.Structure):
+    _fields_ = [("xlon0", C.c_double), ("ylat0", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nx", C.c_uint32), ("ny", C.c_uint32)]
+
+
+class Release(C.Structure):
+    _fields_ = [("lon_min", C.c_double), ("lon_max", C.c_double), ("lat_min", C.c_double),
+                ("lat_max", C.c_double), ("z_min", C.c_double), ("z_max", C.c_double),
+                ("mass_kg", C.c_double), ("particle_count", C.c_uint64),
+                ("start_s", C.c_int64), ("end_s", C.c_int64),
+                ("release_point", C.c_int32), ("class_id", C.c_int32)]
+
+
 class AdjDesc(C.Structure):
     _fields_ = [("n_sources", C.c_uint32), ("met_nx", C.c_uint32), ("met_ny", C.c_uint32),
                 ("met_nz", C.c_uint32), ("n_met_levels", C.c_uint32), ("n_modes", C.c_uint32),
                 ("obs_nz", C.c_uint32), ("obs_dz_m", C.c_float), ("n_steps", C.c_uint32),
                 ("obs_first_step", C.c_uint32), ("checkpoint_every", C.c_uint32),
-                ("reserved", C.c_uint32), ("particles_per_source", C.POINTER(C.c_uint32))]
+                ("flags", C.c_uint32), ("particles_per_source", C.POINTER(C.c_uint32))]
+
+
+ADJ_COMPACT, ADJ_EMISSION_ONLY = 1, 2
 
 
 _f = C.POINTER(C.c_float)
@@ -164,6 +180,8 @@ ABI = {
     "fp_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "fp_autotune": (C.c_int, [_vp, C.POINTER(StepParams), C.POINTER(AutotuneParams),
                               C.POINTER(AutotuneResult)]),
+    "fp_release_inject": (C.c_int, [_vp, C.POINTER(GridDomain), C.POINTER(Release), C.c_uint32, _u64,
+                                    C.c_int64, _u32, C.c_uint64, C.c_uint64, _u64]),
     "fp_interpolation_alpha": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                          C.POINTER(C.c_float)]),
     "fp_adj_create": (C.c_int, [C.c_int, C.POINTER(AdjDesc), C.POINTER(_vp)]),
@@ -176,6 +194,7 @@ ABI = {
     "fp_adj_forward": (C.c_int, [_vp, C.POINTER(StepParams), _f, _f, _f, _d, _d]),
     "fp_adj_backward": (C.c_int, [_vp, _d, _d, _d]),
     "fp_adj_download_state": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fp_adj_active_counts": (C.c_int, [_vp, _u64]),
     "fp_adj_last_ms": (C.c_int, [_vp, _d, _d]),
     "fp_ens_create": (C.c_int, [C.c_int, C.POINTER(EnsDesc), C.POINTER(_vp)]),
     "fp_ens_destroy": (C.c_int, [_vp]),
@@ -392,6 +411,29 @@ class GpuTimestep:
                                         None if wu is None else wu.ctypes.data_as(_d),
                                         None if wv is None else wv.ctypes.data_as(_d)))
         return (li, wu, wv) if want_wind else li
+
+    # -- source-term injection (ReleaseManager::inject_for_time)
+    def inject_releases(self, grid: dict, releases: list, released, now_s: int, slots=None,
+                        first_slot: int = 0) -> int:
+        """Emit the particles the releases' windows owe by now_s straight into
+        the store (fp_release_inject).  grid: GridDomain fields; releases:
+        dicts of Release fields; released: uint64 array of emitted counts,
+        advanced in place.  Returns the number of particles written."""
+        g = GridDomain(**grid)
+        arr = (Release * max(len(releases), 1))()
+        for i, r in enumerate(releases):
+            arr[i] = Release(**r)
+        rel = np.ascontiguousarray(np.asarray(released, np.uint64))
+        sl = None if slots is None else np.ascontiguousarray(np.asarray(slots, np.uint32))
+        n = C.c_uint64()
+        rc = self._lib.fp_release_inject(self._h, C.byref(g), arr, len(releases), rel.ctypes.data_as(_u64),
+                                         int(now_s), None if sl is None else sl.ctypes.data_as(_u32),
+                                         0 if sl is None else len(sl), int(first_slot), C.byref(n))
+        if isinstance(released, np.ndarray):
+            released[:] = rel
+        self.last_injected = int(n.value)
+        self._chk(rc)
+        return int(n.value)
 
     # -- concentration grid
     def configure_grid(self, outheights=None, refine=(1, 1), species=0, mass_scale=1.0e6):
@@ -653,12 +695,14 @@ class GpuAdjoint:
 
     def __init__(self, particles_per_source, met_shape, n_met_levels: int = 2, n_modes: int = 0,
                  obs_nz: int = 11, obs_dz_m: float = 500.0, n_steps: int = 72,
-                 obs_first_step: int = 1, checkpoint_every: int = 8, device: int = 0):
+                 obs_first_step: int = 1, checkpoint_every: int = 8, compact: bool = False,
+                 emission_only: bool = False, device: int = 0):
         lib = library()
         counts = np.ascontiguousarray(np.asarray(particles_per_source, np.uint32))
         nx, ny, nz = met_shape
+        flags = (ADJ_COMPACT if compact else 0) | (ADJ_EMISSION_ONLY if emission_only else 0)
         d = AdjDesc(len(counts), nx, ny, nz, n_met_levels, n_modes, obs_nz, obs_dz_m, n_steps,
-                    obs_first_step, checkpoint_every, 0, counts.ctypes.data_as(_u32))
+                    obs_first_step, checkpoint_every, flags, counts.ctypes.data_as(_u32))
         h = _vp()
         rc = lib.fp_adj_create(device, C.byref(d), C.byref(h))
         if rc != FP_OK:
@@ -671,6 +715,7 @@ class GpuAdjoint:
         self.met_shape = (nx, ny, nz)
         self.n_met_levels = n_met_levels
         self.n_modes = n_modes
+        self.n_steps = n_steps
         self.n_receptors = 0
 
     def _chk(self, rc):
@@ -744,6 +789,13 @@ class GpuAdjoint:
         self._chk(self._lib.fp_adj_backward(self._h, gq.ctypes.data_as(_d), gr.ctypes.data_as(_d),
                                             gc.ctypes.data_as(_d)))
         return gq, gr, gc[: self.n_modes]
+
+    def active_counts(self) -> np.ndarray:
+        """Active particles at the start of every step and after the last
+        (n_steps + 1 values) of the last forward run."""
+        out = np.zeros(self.n_steps + 1, np.uint64)
+        self._chk(self._lib.fp_adj_active_counts(self._h, out.ctypes.data_as(_u64)))
+        return out
 
     def download_state(self, n=None) -> np.ndarray:
         n = self.n_particles if n is None else int(n)

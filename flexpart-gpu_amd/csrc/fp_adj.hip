@@ -24,6 +24,7 @@
 // regime and branch switches, deposition thresholds) contribute nothing: the
 // result is the pathwise derivative.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -589,10 +590,106 @@ __global__ void __launch_bounds__(256) fill_f32_kernel(float *p, uint64_t n, flo
     if (i < n) p[i] = v;
 }
 
+// zero rows 0..nf-1 of a quantity-major buffer over its first n columns
+__global__ void __launch_bounds__(256) zero_rows_kernel(float *p, uint64_t stride, uint32_t nf, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n * nf) return;
+    p[(i / n) * stride + i % n] = 0.0f;
+}
+
 __global__ void __launch_bounds__(256) fill_f64_kernel(double *p, uint64_t n, double v)
 {
     const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// the state fields of survivor j <- slot map[j] (replay of a saved compaction)
+__global__ void __launch_bounds__(256) gather_state_kernel(Soa in, Soa out, const uint32_t *map, uint32_t n_out)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (j >= n_out) return;
+    const uint32_t src = map[j];
+#pragma unroll
+    for (uint32_t f : kStateFields) out.u(f)[j] = in.u(f)[src];
+}
+
+// adjoint of a compaction: lam_pre[map[j]] = lam_post[j] (lam_pre zeroed
+// before: removed particles carry no adjoint)
+__global__ void __launch_bounds__(256) uncompact_kernel(const float *post, float *pre, const uint32_t *map,
+                                                        uint32_t n_post, uint64_t stride, uint32_t nf)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (j >= n_post) return;
+    const uint32_t p = map[j];
+    for (uint32_t f = 0; f < nf; ++f) pre[f * stride + p] = post[f * stride + j];
+}
+
+// Survival factor of the mass over a step (dry and wet deposition), from the
+// particle's state after the step: the final height and the PBL cell of the
+// post-advection position (the step kernel's deposition, same operations).
+__device__ __forceinline__ float survival_factor(const StepArgs &a, const Soa &s, uint64_t j)
+{
+    float fm = 1.0f;
+    const float zf = s.f(F_POS_Z)[j];
+    if (a.dry_mode != FP_DRY_OFF) {
+        const float href = fmaxf(a.dry_href, 0.1f);
+        if (!(zf >= 2.0f * href)) {
+            const float ex = -fmaxf(a.dry_vd, 0.0f) * fabsf(a.dt) / (2.0f * href);
+            fm = fm * clampf(fpm_expf(ex), 0.0f, 1.0f);
+        }
+    }
+    if (a.wet_mode != FP_WET_OFF) {
+        const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
+        const float hx = (float)s.i(F_CELL_X)[j] + s.f(F_POS_X)[j], hy = (float)s.i(F_CELL_Y)[j] + s.f(F_POS_Y)[j];
+        const uint32_t pidx = (uint32_t)floorf(clampf(hx, 0.0f, nxm)) * a.ny + (uint32_t)floorf(clampf(hy, 0.0f, nym));
+        float lam, frac;
+        if (a.wet_mode == FP_WET_PRECIP_FIELD) {
+            const float pr = pos_or_zero(a.precip[pidx]);
+            lam = 0.0f;
+            if (pr > 0.0f) lam = pos_or_zero(pos_or_zero(a.wet_a) * fpm_powf(pr, pos_or_zero(a.wet_b)));
+            frac = a.wet_fraction;
+        } else {
+            lam = a.wet_lambda;
+            frac = a.wet_fraction;
+        }
+        const float l = fmaxf(lam, 0.0f), dt = fabsf(a.dt), fr = clampf(frac, 0.0f, 1.0f);
+        float prob = 0.0f;
+        if (!(l <= 0.0f || dt <= 0.0f || fr <= 0.0f)) prob = clampf((1.0f - fpm_expf(-l * dt)) * fr, 0.0f, 1.0f);
+        fm = fm * (1.0f - prob);
+    }
+    return fm;
+}
+
+// Mass adjoint of one step (emission gradients only).  For every particle j
+// alive after step k (post state, post-compaction index): the observation at
+// time k + 1 adds dC/dm, then the step's survival factor maps the mass
+// adjoint to its slot at the start of the step (map[j], or j without
+// compaction).
+__global__ void __launch_bounds__(256) mass_adj_kernel(StepArgs a, Soa post, uint32_t n_post, const uint32_t *map,
+                                                       bool observe, ObsArgs o, const double *rec_lam,
+                                                       const float *lam_post, float *lam_pre)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (j >= n_post) return;
+    const uint32_t p = map ? map[j] : (uint32_t)j;
+    const bool act = (post.u(F_FLAGS)[j] & FLAG_ACTIVE) != 0u;
+    float lm = lam_post[j];
+    if (act && observe) {
+        const Cic c = cic_of(o, (float)post.i(F_CELL_X)[j] + post.f(F_POS_X)[j],
+                             (float)post.i(F_CELL_Y)[j] + post.f(F_POS_Y)[j], post.f(F_POS_Z)[j]);
+        float g = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t rec = o.node_rec[cic_node(o, c, k)];
+            if (rec >= 0) g += cic_weight(c, k) * (float)rec_lam[rec];
+        }
+        lm += g;
+    }
+    // a particle inactive after the step either was inactive before it
+    // (untouched: the adjoint passes through) or was deactivated by it (held
+    // fixed: it carries no adjoint to earlier steps)
+    lam_pre[p] = act ? lm * survival_factor(a, post, j) : (map ? 0.0f : lm);
 }
 
 uint32_t grid_for(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
@@ -617,6 +714,7 @@ struct fp_adj {
     hipStream_t stream = nullptr;
     std::string err;
     fp_adj_desc d{};
+    bool compact = false, emission_only = false;
     std::vector<uint32_t> counts;
     std::vector<uint64_t> offs;
     uint64_t n = 0, npts = 0, nxy = 0;
@@ -639,13 +737,20 @@ struct fp_adj {
     uint32_t *src_of = nullptr, *d_counts = nullptr;
     uint64_t *d_offs = nullptr;
     float *d_q = nullptr, *d_r = nullptr, *d_c = nullptr;
-    std::vector<float> r_host;
 
-    fp::Soa work;
+    fp::Soa work, work2;               // forward state; compaction target and backward scratch
     std::vector<uint32_t> ckpt_steps;  // steps whose state is kept (0, K, 2K, ..., n_steps)
     std::vector<fp::Soa> ckpt;
     std::vector<fp::Soa> tape;         // K - 1 sets
-    float *lam = nullptr;
+    float *lam = nullptr, *lam2 = nullptr;
+
+    // per step of the last forward: active count at its start, Philox counter,
+    // survivor map of the compaction after it
+    std::vector<uint32_t> step_n;
+    std::vector<std::array<uint32_t, 4>> step_ctr;
+    std::vector<uint64_t> map_off;
+    uint32_t *maps = nullptr;
+    uint32_t *block_counts = nullptr, *active_dev = nullptr, *active_host = nullptr;
 
     int32_t *node_rec = nullptr;
     uint64_t n_nodes = 0;
@@ -705,10 +810,12 @@ void destroy(fp_adj *e)
     fr(e->basis_u); fr(e->basis_v); fr(e->pert_u); fr(e->pert_v);
     fr(e->pbl); fr(e->precip); fr(e->pbl_stage);
     fr(e->src_of); fr(e->d_counts); fr(e->d_offs); fr(e->d_q); fr(e->d_r); fr(e->d_c);
-    fr(e->work.base);
+    fr(e->work.base); fr(e->work2.base);
     for (auto &s : e->ckpt) fr(s.base);
     for (auto &s : e->tape) fr(s.base);
-    fr(e->lam); fr(e->node_rec); fr(e->rec_val); fr(e->rec_lam);
+    fr(e->lam); fr(e->lam2); fr(e->maps); fr(e->block_counts); fr(e->active_dev);
+    if (e->active_host) (void)hipHostFree(e->active_host);
+    fr(e->node_rec); fr(e->rec_val); fr(e->rec_lam);
     fr(e->au); fr(e->av); fr(e->gq); fr(e->gr); fr(e->gc); fr(e->aos_stage);
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
@@ -716,22 +823,23 @@ void destroy(fp_adj *e)
     delete e;
 }
 
-// copy the fields a step reads and writes (one species)
-hipError_t copy_state(fp_adj *e, const fp::Soa &dst, const fp::Soa &src)
+// copy the fields a step reads and writes (one species) of the first cnt slots
+hipError_t copy_state(fp_adj *e, const fp::Soa &dst, const fp::Soa &src, uint64_t cnt)
 {
+    if (cnt == 0) return hipSuccess;
     for (uint32_t f : fp::kStateFields) {
-        const hipError_t h = hipMemcpyAsync(dst.u(f), src.u(f), e->n * 4, hipMemcpyDeviceToDevice, e->stream);
+        const hipError_t h = hipMemcpyAsync(dst.u(f), src.u(f), cnt * 4u, hipMemcpyDeviceToDevice, e->stream);
         if (h != hipSuccess) return h;
     }
     return hipSuccess;
 }
 
-// the step arguments of step k on state s
+// the step arguments of step k on state s (count and counter of the last forward)
 fp::StepArgs step_args(const fp_adj *e, uint32_t k, const fp::Soa &s)
 {
     fp::StepArgs a{};
     a.s = s;
-    a.n = (uint32_t)e->n;
+    a.n = e->step_n[k];
     a.n_species = 1;
     const double t = (double)k * (double)e->params.dt_seconds;
     const uint32_t nb = e->d.n_met_levels - 1u;
@@ -749,7 +857,7 @@ fp::StepArgs step_args(const fp_adj *e, uint32_t k, const fp::Soa &s)
     // interpolation_alpha over the bracket (timeloop.rs:2086-2120), clamped
     const double t0 = (double)b * e->level_seconds;
     p.alpha = (float)((t - t0) / e->level_seconds);
-    fp::counter_advance(e->params.counter, (uint64_t)k * fp::counter_per_step(e->n, p.n_substeps), p.counter);
+    for (int c = 0; c < 4; ++c) p.counter[c] = e->step_ctr[k][c];
     p.grid_fused = 0;
     fp::fill_step_scalars(a, p);
     a.ppt = 1;
@@ -767,6 +875,8 @@ int ckpt_index(const fp_adj *e, uint32_t step)
         if (e->ckpt_steps[j] == step) return (int)j;
     return -1;
 }
+
+bool observed(const fp_adj *e, uint32_t t) { return t >= e->d.obs_first_step && e->n_rec > 0; }
 
 }  // namespace
 
@@ -787,6 +897,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     if (d.n_steps == 0 || d.checkpoint_every == 0) return afail(nullptr, FP_E_INVALID, "n_steps and checkpoint_every must be >= 1");
     if (d.obs_first_step == 0 || d.obs_first_step > d.n_steps)
         return afail(nullptr, FP_E_INVALID, "obs_first_step must be in [1, n_steps]");
+    if (d.flags & ~(FP_ADJ_COMPACT | FP_ADJ_EMISSION_ONLY)) return afail(nullptr, FP_E_INVALID, "unknown flags");
     uint64_t total = 0;
     for (uint32_t k = 0; k < d.n_sources; ++k) {
         if (d.particles_per_source[k] == 0) return afail(nullptr, FP_E_INVALID, "every source needs >= 1 particle");
@@ -805,6 +916,8 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     e->device = device_ordinal;
     e->d = d;
     e->d.particles_per_source = nullptr;
+    e->compact = (d.flags & FP_ADJ_COMPACT) != 0;
+    e->emission_only = (d.flags & FP_ADJ_EMISSION_ONLY) != 0;
     e->counts.assign(d.particles_per_source, d.particles_per_source + d.n_sources);
     e->offs.resize(d.n_sources);
     uint64_t acc = 0;
@@ -815,6 +928,9 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     for (uint32_t k = 0; k < d.n_steps; k += d.checkpoint_every) e->ckpt_steps.push_back(k);
     e->ckpt_steps.push_back(d.n_steps);
     e->n_nodes = e->nxy * d.obs_nz;
+    e->step_n.assign(d.n_steps + 1u, 0u);
+    e->step_ctr.assign(d.n_steps + 1u, std::array<uint32_t, 4>{0u, 0u, 0u, 0u});
+    e->map_off.assign(d.n_steps + 1u, 0u);
 
     auto bail = [&](hipError_t h, const char *what) {
         g_adj_error = std::string(what) + ": " + hipGetErrorString(h);
@@ -855,11 +971,19 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
         return amalloc(&s.base, total * fp::F_COUNT);
     };
     C_HIP(soa_alloc(e->work));
+    C_HIP(soa_alloc(e->work2));
     e->ckpt.resize(e->ckpt_steps.size());
     for (auto &s : e->ckpt) C_HIP(soa_alloc(s));
     e->tape.resize(d.checkpoint_every > 1 ? d.checkpoint_every - 1u : 0u);
     for (auto &s : e->tape) C_HIP(soa_alloc(s));
     C_HIP(amalloc(&e->lam, total * fp::kLam));
+    C_HIP(amalloc(&e->lam2, total * fp::kLam));
+    if (e->compact) {
+        C_HIP(amalloc(&e->maps, total * (uint64_t)d.n_steps));
+        C_HIP(amalloc(&e->block_counts, fp::compact_blocks((uint32_t)total) + 1u));
+        C_HIP(amalloc(&e->active_dev, 1));
+        C_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->active_host), 4, hipHostMallocDefault));
+    }
     C_HIP(amalloc(&e->node_rec, e->n_nodes));
     C_HIP(amalloc(&e->rec_val, e->n_nodes));
     C_HIP(amalloc(&e->rec_lam, e->n_nodes));
@@ -867,7 +991,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     C_HIP(amalloc(&e->av, e->nxy));
     C_HIP(amalloc(&e->aos_stage, total));
     // zero every particle set once so that fields the step never writes read as zero
-    C_HIP(hipMemsetAsync(e->work.base, 0, total * fp::F_COUNT * 4u, e->stream));
+    for (fp::Soa *s : {&e->work, &e->work2}) C_HIP(hipMemsetAsync(s->base, 0, total * fp::F_COUNT * 4u, e->stream));
     for (auto &s : e->ckpt) C_HIP(hipMemsetAsync(s.base, 0, total * fp::F_COUNT * 4u, e->stream));
     for (auto &s : e->tape) C_HIP(hipMemsetAsync(s.base, 0, total * fp::F_COUNT * 4u, e->stream));
     C_HIP(hipMemsetAsync(e->node_rec, 0xff, e->n_nodes * 4u, e->stream));
@@ -984,13 +1108,12 @@ int fp_adj_forward(fp_adj *e, const fp_step_params *p, const float *q, const flo
     A_CHECK(e, p->dry_mode <= FP_DRY_UNIFORM, "differentiable runs support dry_mode off or uniform");
     A_CHECK(e, p->wet_mode != FP_WET_PER_SLOT && p->wet_mode <= FP_WET_PRECIP_FIELD, "unsupported wet_mode");
     A_STATE(e, p->wet_mode != FP_WET_PRECIP_FIELD || e->precip_ready, "precip field missing");
-    A_CHECK(e, p->loss_mode == FP_LOSS_NONE, "differentiable runs keep every particle (loss_mode none)");
+    A_CHECK(e, p->loss_mode <= FP_LOSS_MIN_MASS, "unknown loss_mode");
     for (uint32_t k = 0; k < e->d.n_sources; ++k)
         A_CHECK(e, std::isfinite(q[k]) && std::isfinite(r[3 * k]) && std::isfinite(r[3 * k + 1]) &&
                        std::isfinite(r[3 * k + 2]),
                 "emissions and positions must be finite");
     e->params = *p;
-    e->r_host.assign(r, r + 3u * e->d.n_sources);
     const uint32_t S = e->d.n_sources, K = e->d.n_modes;
     hipStream_t st = e->stream;
     A_HIP(e, hipMemcpyAsync(e->d_q, q, S * 4u, hipMemcpyHostToDevice, st));
@@ -1010,15 +1133,33 @@ int fp_adj_forward(fp_adj *e, const fp_step_params *p, const float *q, const flo
                                                                e->d_q, e->d_r, e->d.met_nx, e->d.met_ny);
     fp::fill_f64_kernel<<<fp::grid_for(std::max<uint64_t>(e->n_rec, 1)), 256, 0, st>>>(e->rec_val, e->n_rec, 0.0);
     A_HIP(e, hipGetLastError());
-    A_HIP(e, copy_state(e, e->ckpt[0], e->work));
+    A_HIP(e, copy_state(e, e->ckpt[0], e->work, e->n));
     const fp::ObsArgs oa = obs_args(e);
+    e->step_n[0] = (uint32_t)e->n;
+    for (int cc = 0; cc < 4; ++cc) e->step_ctr[0][cc] = p->counter[cc];
+    uint64_t moff = 0;
     for (uint32_t k = 0; k < e->d.n_steps; ++k) {
+        const uint32_t nk = e->step_n[k];
         const fp::StepArgs a = step_args(e, k, e->work);
         A_HIP(e, fp::launch_step(a, 256, st));
-        if (k + 1u >= e->d.obs_first_step && e->n_rec)
-            fp::obs_deposit_kernel<<<fp::grid_for(e->n), 256, 0, st>>>(e->work, (uint32_t)e->n, oa, e->rec_val);
+        if (observed(e, k + 1u) && nk)
+            fp::obs_deposit_kernel<<<fp::grid_for(nk), 256, 0, st>>>(e->work, nk, oa, e->rec_val);
+        uint32_t next = nk;
+        if (e->compact) {
+            // compaction after the step (timeloop.rs:1185-1197); its survivor map is kept
+            e->map_off[k] = moff;
+            A_HIP(e, fp::launch_compact(e->work, e->work2, nk, e->block_counts, e->maps + moff, e->active_dev, st));
+            A_HIP(e, hipMemcpyAsync(e->active_host, e->active_dev, 4, hipMemcpyDeviceToHost, st));
+            A_HIP(e, hipStreamSynchronize(st));
+            next = *e->active_host;
+            moff += next;
+            std::swap(e->work, e->work2);
+        }
+        e->step_n[k + 1u] = next;
+        fp::counter_advance(e->step_ctr[k].data(), fp::counter_per_step(nk, p->n_substeps),
+                            e->step_ctr[k + 1u].data());
         const int j = ckpt_index(e, k + 1u);
-        if (j >= 0) A_HIP(e, copy_state(e, e->ckpt[j], e->work));
+        if (j >= 0) A_HIP(e, copy_state(e, e->ckpt[j], e->work, next));
     }
     A_HIP(e, hipGetLastError());
     A_HIP(e, hipEventRecord(e->e1, st));
@@ -1045,42 +1186,79 @@ int fp_adj_backward(fp_adj *e, double *grad_q, double *grad_r, double *grad_c)
     A_CHECK(e, e, "null argument");
     A_STATE(e, e->forward_done, "fp_adj_backward before fp_adj_forward");
     hipStream_t st = e->stream;
-    const uint32_t S = e->d.n_sources, K = e->d.n_modes, ce = e->d.checkpoint_every;
+    const uint32_t S = e->d.n_sources, K = e->d.n_modes;
+    const bool full = !e->emission_only;
+    const uint32_t nf = full ? fp::kLam : 1u;   // adjoint quantities carried
+    // the mass adjoint lives in row L_M of the adjoint buffers
     std::vector<double> lamr(e->n_rec);
     for (uint32_t k = 0; k < e->n_rec; ++k) lamr[k] = 2.0 * (e->val_host[k] - e->obs_host[k]);
     if (e->n_rec)
         A_HIP(e, hipMemcpyAsync(e->rec_lam, lamr.data(), e->n_rec * 8u, hipMemcpyHostToDevice, st));
     A_HIP(e, hipEventRecord(e->e0, st));
     fp::fill_f32_kernel<<<fp::grid_for(e->n * fp::kLam), 256, 0, st>>>(e->lam, e->n * fp::kLam, 0.0f);
-    fp::fill_f64_kernel<<<fp::grid_for(e->nxy), 256, 0, st>>>(e->au, e->nxy, 0.0);
-    fp::fill_f64_kernel<<<fp::grid_for(e->nxy), 256, 0, st>>>(e->av, e->nxy, 0.0);
+    if (full && K) {
+        fp::fill_f64_kernel<<<fp::grid_for(e->nxy), 256, 0, st>>>(e->au, e->nxy, 0.0);
+        fp::fill_f64_kernel<<<fp::grid_for(e->nxy), 256, 0, st>>>(e->av, e->nxy, 0.0);
+    }
     A_HIP(e, hipGetLastError());
     const fp::ObsArgs oa = obs_args(e);
+    float *lam = e->lam, *lam2 = e->lam2;
+    const uint64_t stride = e->n;
     const uint32_t nseg = (uint32_t)e->ckpt_steps.size() - 1u;
     for (uint32_t seg = nseg; seg-- > 0;) {
         const uint32_t k0 = e->ckpt_steps[seg], k1 = e->ckpt_steps[seg + 1u];
         // recompute the segment, keeping the state at the start of each step
+        // (after the previous step's compaction, replayed from its saved map)
         for (uint32_t k = k0 + 1u; k < k1; ++k) {
             const fp::Soa &prev = (k - 1u == k0) ? e->ckpt[seg] : e->tape[k - 2u - k0];
             const fp::Soa &cur = e->tape[k - 1u - k0];
-            A_HIP(e, copy_state(e, cur, prev));
-            A_HIP(e, fp::launch_step(step_args(e, k - 1u, cur), 256, st));
+            if (e->compact) {
+                A_HIP(e, copy_state(e, e->work2, prev, e->step_n[k - 1u]));
+                A_HIP(e, fp::launch_step(step_args(e, k - 1u, e->work2), 256, st));
+                if (e->step_n[k])
+                    fp::gather_state_kernel<<<fp::grid_for(e->step_n[k]), 256, 0, st>>>(
+                        e->work2, cur, e->maps + e->map_off[k - 1u], e->step_n[k]);
+            } else {
+                A_HIP(e, copy_state(e, cur, prev, e->step_n[k - 1u]));
+                A_HIP(e, fp::launch_step(step_args(e, k - 1u, cur), 256, st));
+            }
         }
         for (uint32_t k = k1; k-- > k0;) {
-            // observation at time k + 1 (the state after step k)
-            if (k + 1u >= e->d.obs_first_step && e->n_rec) {
-                const fp::Soa &after = (k + 1u == k1) ? e->ckpt[seg + 1u] : e->tape[k - k0];
-                fp::obs_adjoint_kernel<<<fp::grid_for(e->n), 256, 0, st>>>(after, (uint32_t)e->n, oa, e->rec_lam,
-                                                                          e->lam, e->n);
-            }
+            const fp::Soa &after = (k + 1u == k1) ? e->ckpt[seg + 1u] : e->tape[k - k0];
             const fp::Soa &before = (k == k0) ? e->ckpt[seg] : e->tape[k - 1u - k0];
-            A_HIP(e, fp::launch_adj_step(step_args(e, k, before), e->lam, e->n, K ? e->au : nullptr,
+            const uint32_t n_after = e->step_n[k + 1u], n_before = e->step_n[k];
+            const uint32_t *map = e->compact ? e->maps + e->map_off[k] : nullptr;
+            if (!full) {
+                // observation at k + 1, survivor map and survival factor in one pass
+                if (map) fp::fill_f32_kernel<<<fp::grid_for(n_before), 256, 0, st>>>(lam2 + fp::L_M * stride, n_before, 0.0f);
+                const fp::StepArgs a = step_args(e, k, after);
+                if (n_after)
+                    fp::mass_adj_kernel<<<fp::grid_for(n_after), 256, 0, st>>>(
+                        a, after, n_after, map, observed(e, k + 1u), oa, e->rec_lam, lam + fp::L_M * stride,
+                        (map ? lam2 : lam) + fp::L_M * stride);
+                if (map) std::swap(lam, lam2);
+                continue;
+            }
+            // observation at time k + 1 (the state after step k; compaction
+            // keeps every observed particle, so the post-compaction state serves)
+            if (observed(e, k + 1u) && n_after)
+                fp::obs_adjoint_kernel<<<fp::grid_for(n_after), 256, 0, st>>>(after, n_after, oa, e->rec_lam, lam,
+                                                                              stride);
+            if (map) {
+                if (n_before)
+                    fp::zero_rows_kernel<<<fp::grid_for((uint64_t)n_before * nf), 256, 0, st>>>(lam2, stride, nf,
+                                                                                                n_before);
+                if (n_after)
+                    fp::uncompact_kernel<<<fp::grid_for(n_after), 256, 0, st>>>(lam, lam2, map, n_after, stride, nf);
+                std::swap(lam, lam2);
+            }
+            A_HIP(e, fp::launch_adj_step(step_args(e, k, before), lam, stride, K ? e->au : nullptr,
                                          K ? e->av : nullptr, st));
         }
     }
-    fp::source_grad_kernel<<<S, 256, 0, st>>>(e->lam, e->n, e->d_offs, e->d_counts, e->d_r, e->d.met_nx,
+    fp::source_grad_kernel<<<S, 256, 0, st>>>(lam, stride, e->d_offs, e->d_counts, e->d_r, e->d.met_nx,
                                               e->d.met_ny, e->gq, e->gr);
-    if (K)
+    if (full && K)
         fp::coeff_grad_kernel<<<K, 256, 0, st>>>(e->au, e->av, e->basis_u, e->basis_v, (uint32_t)e->nxy, e->gc);
     A_HIP(e, hipGetLastError());
     A_HIP(e, hipEventRecord(e->e1, st));
@@ -1088,10 +1266,14 @@ int fp_adj_backward(fp_adj *e, double *grad_q, double *grad_r, double *grad_c)
     if (grad_r) A_HIP(e, hipMemcpyAsync(grad_r, e->gr, S * 24u, hipMemcpyDeviceToHost, st));
     if (grad_c && K) A_HIP(e, hipMemcpyAsync(grad_c, e->gc, K * 8u, hipMemcpyDeviceToHost, st));
     A_HIP(e, hipStreamSynchronize(st));
+    if (!full) {
+        // position and wind gradients are not computed in this mode
+        if (grad_r) std::fill(grad_r, grad_r + 3u * S, 0.0);
+        if (grad_c) std::fill(grad_c, grad_c + K, 0.0);
+    }
     float ms = 0.0f;
     A_HIP(e, hipEventElapsedTime(&ms, e->e0, e->e1));
     e->bwd_ms = ms;
-    (void)ce;
     return FP_OK;
 }
 
@@ -1104,6 +1286,14 @@ int fp_adj_download_state(fp_adj *e, fp_particle *out, uint64_t n)
     A_HIP(e, fp::launch_soa_to_aos(e->work, e->aos_stage, n, e->stream));  // the backward pass leaves it
     A_HIP(e, hipMemcpyAsync(out, e->aos_stage, n * sizeof(fp_particle), hipMemcpyDeviceToHost, e->stream));
     A_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+int fp_adj_active_counts(fp_adj *e, uint64_t *counts)
+{
+    A_CHECK(e, e && counts, "null argument");
+    A_STATE(e, e->forward_done, "fp_adj_active_counts before fp_adj_forward");
+    for (uint32_t k = 0; k <= e->d.n_steps; ++k) counts[k] = e->step_n[k];
     return FP_OK;
 }
 

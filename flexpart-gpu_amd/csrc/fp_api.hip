@@ -17,6 +17,8 @@ static_assert(sizeof(fp_grid_params) == 80, "fp_grid_params layout is part of th
 static_assert(sizeof(fp_sort_params) == 56, "fp_sort_params layout is part of the ABI");
 static_assert(sizeof(fp_autotune_params) == 64, "fp_autotune_params layout is part of the ABI");
 static_assert(sizeof(fp_autotune_result) == 32, "fp_autotune_result layout is part of the ABI");
+static_assert(sizeof(fp_grid_domain) == 40, "fp_grid_domain layout is part of the ABI");
+static_assert(sizeof(fp_release) == 88, "fp_release layout is part of the ABI");
 
 namespace {
 
@@ -141,7 +143,7 @@ void destroy(fp_ctx *ctx)
     fr(ctx->block_counts); fr(ctx->survivor_map); fr(ctx->active_count_dev);
     fr(ctx->bounds_partial); fr(ctx->bounds_dev);
     fr(ctx->sort_keys); fr(ctx->sort_keys_alt); fr(ctx->sort_vals); fr(ctx->sort_vals_alt);
-    fr(ctx->sort_temp);
+    fr(ctx->sort_temp); fr(ctx->release_jobs);
     if (ctx->active_count_host) (void)hipHostFree(ctx->active_count_host);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     for (auto &ev : ctx->free_events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
@@ -695,6 +697,54 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     FP_HIP(ctx, e);
+    return FP_OK;
+}
+
+int fp_release_inject(fp_ctx *ctx, const fp_grid_domain *grid, const fp_release *rel, uint32_t n_rel,
+                      uint64_t *released, int64_t now_s, const uint32_t *slots, uint64_t n_slots,
+                      uint64_t first_slot, uint64_t *injected)
+{
+    FP_CHECK(ctx, ctx && grid && (rel || n_rel == 0) && (released || n_rel == 0), "null argument");
+    if (injected) *injected = 0;
+    FP_CHECK(ctx, grid->nx >= 1 && grid->ny >= 1 && grid->dx != 0.0 && grid->dy != 0.0, "invalid grid domain");
+    FP_CHECK(ctx, now_s >= INT32_MIN && now_s <= INT32_MAX, "timestamp cannot be represented as i32 seconds");
+    for (uint32_t k = 0; k < n_rel; ++k) {
+        FP_CHECK(ctx, rel[k].particle_count >= 1, "release particle_count must be >= 1");
+        FP_CHECK(ctx, rel[k].lon_min <= rel[k].lon_max, "invalid bounds: longitude min > max");
+        FP_CHECK(ctx, rel[k].lat_min <= rel[k].lat_max, "invalid bounds: latitude min > max");
+        FP_CHECK(ctx, rel[k].z_min <= rel[k].z_max, "invalid bounds: height min > max");
+    }
+    const uint64_t cap = ctx->desc.capacity;
+    uint64_t max_particles;
+    const uint32_t *slots_dev = nullptr;
+    if (slots) {
+        for (uint64_t k = 0; k < n_slots; ++k) FP_CHECK(ctx, slots[k] < cap, "slot index out of range");
+        max_particles = n_slots;
+        if (n_slots) {
+            uint64_t lo = UINT64_MAX, hi = 0;
+            for (uint64_t k = 0; k < n_slots; ++k) {
+                lo = std::min<uint64_t>(lo, slots[k]);
+                hi = std::max<uint64_t>(hi, slots[k]);
+            }
+            if (int rc = ensure_current(ctx, lo, hi + 1)) return rc;
+            if (int rc = ensure_slot_stage(ctx, n_slots)) return rc;
+            FP_HIP(ctx, hipMemcpyAsync(ctx->slot_stage, slots, n_slots * 4, hipMemcpyHostToDevice, ctx->stream));
+            slots_dev = ctx->slot_stage;
+        }
+    } else {
+        FP_CHECK(ctx, first_slot <= cap, "first_slot beyond capacity");
+        max_particles = cap - first_slot;
+    }
+    uint64_t n = 0;
+    int status = 0;
+    FP_HIP(ctx, fp::release_plan_and_launch(*grid, rel, n_rel, released, now_s, (int32_t)now_s, slots_dev,
+                                            first_slot, max_particles, ctx->s(), &ctx->release_jobs,
+                                            &ctx->release_jobs_bytes, ctx->stream, &n, &status));
+    // every word of the written slots is new: the current set owns them now
+    if (!slots) ctx->segs.assign(first_slot, first_slot + n, ctx->cur);
+    if (injected) *injected = n;
+    if (status == 1) return fail(ctx, FP_E_INVALID, "release sampled outside the grid domain");
+    if (status == 2) return fail(ctx, FP_E_INVALID, "particle store full");
     return FP_OK;
 }
 

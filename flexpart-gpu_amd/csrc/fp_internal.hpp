@@ -188,6 +188,10 @@ struct fp_ctx {
     void *sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
 
+    // source-term injection plans
+    void *release_jobs = nullptr;
+    uint64_t release_jobs_bytes = 0;
+
     // launch shape
     uint32_t block_size = 256;
     uint32_t ppt = 1;
@@ -224,6 +228,11 @@ hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2]
                                 uint32_t *out, hipStream_t st);
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
                              hipStream_t st);
+// Source-term injection (fp_release.hip).
+hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *rel, uint32_t n_rel,
+                                   uint64_t *released, int64_t now, int32_t time, const uint32_t *slots_dev,
+                                   uint64_t first_slot, uint64_t max_particles, const Soa &s, void **jobs_dev,
+                                   uint64_t *jobs_cap_bytes, hipStream_t st, uint64_t *injected, int *status);
 // Adjoint of one step (fp_adj.hip): lam (7 x stride floats: x, y, z, u', v',
 // w', m0 of the step's output) is replaced by the adjoint of its input; au/av
 // (nx*ny doubles, may be null) accumulate the adjoint of the horizontal wind.

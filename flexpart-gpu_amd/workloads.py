@@ -126,12 +126,13 @@ def point_release(n_particles: int = 100_000, seed: int = 42) -> Workload:
                     dry_vd=2.0e-3, release_seconds=3600.0)
 
 
-def _fourier_field(rng, nx, ny, kmax, rms):
+def _fourier_field(rng, nx, ny, kmax, rms, deg: float = 1.0):
     """sum over 0 <= kx, ky <= kmax, (kx, ky) != 0 of a cos(kx lon + 2 ky lat + ph),
-    a ~ N(0, 1), ph ~ U[0, 2 pi) drawn in (kx, ky) order, scaled to the given rms.
+    a ~ N(0, 1), ph ~ U[0, 2 pi) drawn in (kx, ky) order, scaled to the given rms,
+    on a global grid of ``deg`` degree spacing (lon from 0, lat from -90).
     Evaluated as Re(Ex^T C Ey) with Ex = exp(i kx lon), Ey = exp(2 i ky lat)."""
-    lon = np.deg2rad(np.arange(nx, dtype=np.float64))
-    lat = np.deg2rad(np.arange(ny, dtype=np.float64) - 90.0)
+    lon = np.deg2rad(np.arange(nx, dtype=np.float64) * deg)
+    lat = np.deg2rad(np.arange(ny, dtype=np.float64) * deg - 90.0)
     k = np.arange(kmax + 1)
     coef = np.zeros((kmax + 1, kmax + 1), np.complex128)
     for kx in range(0, kmax + 1):
@@ -148,33 +149,34 @@ def _fourier_field(rng, nx, ny, kmax, rms):
     return f
 
 
-def _wind_level(rng, nx, ny, lh):
-    lat = np.deg2rad(np.arange(ny, dtype=np.float64) - 90.0)
-    up = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0))
-    vp = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0))
+def _wind_level(rng, nx, ny, lh, deg: float = 1.0):
+    lat = np.deg2rad(np.arange(ny, dtype=np.float64) * deg - 90.0)
+    up = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0), deg)
+    vp = _fourier_field(rng, nx, ny, 8, 5.0 / math.sqrt(2.0), deg)
     u2 = 30.0 * np.sin(2.0 * lat)[None, :] + up
     v2 = vp
     # horizontal divergence on the sphere (centred differences, periodic in lon)
     coslat = np.maximum(np.cos(lat), 1e-3)[None, :]
-    dx = METRES_PER_DEG * coslat
+    dx = METRES_PER_DEG * deg * coslat
     dudx = (np.roll(u2, -1, 0) - np.roll(u2, 1, 0)) / (2.0 * dx)
-    dvdy = np.gradient(v2 * coslat, axis=1) / (METRES_PER_DEG * coslat)
+    dvdy = np.gradient(v2 * coslat, axis=1) / (METRES_PER_DEG * deg * coslat)
     div = dudx + dvdy
     zf = lh.astype(np.float64)
     g = 0.6 + 0.4 * np.minimum(zf / 10_000.0, 1.0)           # wind profile factor
     G = np.concatenate([[0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(zf))])  # int_0^z g
-    u = (u2[:, :, None] * g[None, None, :]).astype(np.float32)
-    v = (v2[:, :, None] * g[None, None, :]).astype(np.float32)
-    w = (-div[:, :, None] * G[None, None, :]).astype(np.float32)  # continuity
+    gf = g.astype(np.float32)[None, None, :]
+    u = u2.astype(np.float32)[:, :, None] * gf
+    v = v2.astype(np.float32)[:, :, None] * gf
+    w = (-div).astype(np.float32)[:, :, None] * G.astype(np.float32)[None, None, :]  # continuity
     return u.reshape(-1), v.reshape(-1), w.reshape(-1)
 
 
 def global_met(seed: int = 2, nx: int = 360, ny: int = 181, nz: int = 60,
-               n_levels: int = 5) -> Met:
+               n_levels: int = 5, deg: float = 1.0, seconds_between_levels: float = 6 * 3600.0) -> Met:
     lh = level_heights(nz, 25_000.0)
-    levels = [_wind_level(np.random.default_rng(seed * 1000 + t), nx, ny, lh)
+    levels = [_wind_level(np.random.default_rng(seed * 1000 + t), nx, ny, lh, deg)
               for t in range(n_levels)]
-    return Met((nx, ny, nz), levels, lh, 6 * 3600.0)
+    return Met((nx, ny, nz), levels, lh, seconds_between_levels)
 
 
 def global_pbl(seed: int = 2, nx: int = 360, ny: int = 181) -> Pbl:
@@ -344,3 +346,54 @@ def adjoint_case(n_particles: int = 2_000_000, n_sources: int = 1024, n_modes: i
     scale = (1.0 / METRES_PER_DEG) if x_scale is None else x_scale
     return AdjointCase(met, pbl, counts, q, pos, bu, bv, np.zeros(n_modes, np.float32), dt_seconds, scale, scale,
                        (seed, 0x0AD10AD1))
+
+
+def precip_patches(nx: int, ny: int, cover: float = 0.2, median: float = 2.0, sigma: float = 1.0,
+                   seed: int = 5) -> np.ndarray:
+    """Precipitation [mm/h] in seeded elliptical cloud patches covering about
+    ``cover`` of the (periodic in x) domain, lognormal inside (median
+    ``median``, log-sd ``sigma``), zero elsewhere; flat index i * ny + j."""
+    rng = np.random.default_rng(seed * 3571)
+    mask = np.zeros((nx, ny), bool)
+    scale = nx / 360.0
+    while mask.mean() < cover:
+        cx, cy = rng.uniform(0, nx), rng.uniform(0, ny - 1)
+        a, b = rng.uniform(4, 30) * scale, rng.uniform(2, 15) * scale
+        th = rng.uniform(0, np.pi)
+        r = int(np.ceil(max(a, b))) + 1
+        ii = np.arange(int(cx) - r, int(cx) + r + 1)
+        jj = np.arange(max(0, int(cy) - r), min(ny, int(cy) + r + 1))
+        dx = (ii[:, None] - cx).astype(np.float64)
+        dy = (jj[None, :] - cy).astype(np.float64)
+        u = (dx * np.cos(th) + dy * np.sin(th)) / a
+        v = (-dx * np.sin(th) + dy * np.cos(th)) / b
+        inside = u * u + v * v <= 1.0
+        mask[np.ix_(ii % nx, jj)] |= inside
+    p = np.where(mask, median * np.exp(sigma * rng.normal(size=(nx, ny))), 0.0)
+    return p.astype(np.float32).reshape(-1)
+
+
+def stress_case(n_particles: int = 50_000_000, n_sources: int = 4096, seed: int = 6, deg: float = 0.25,
+                nz: int = 60) -> AdjointCase:
+    """The 50M-particle deposition stress case: 0.25 deg global met (1440 x
+    721 x 60) with two time levels 24 h apart, precipitation in elliptical
+    patches over 20 % of the domain, 4096 sources at uniform-random positions
+    (heights U[10, 500] m) with lognormal emissions; particles are split in
+    proportion to the emissions so that every particle starts with the same
+    mass (the loss rule then removes a particle once deposition has taken a
+    fixed fraction of it)."""
+    nx, ny = int(round(360 / deg)), int(round(180 / deg)) + 1
+    met = global_met(seed, nx, ny, nz, n_levels=2, deg=deg, seconds_between_levels=24 * 3600.0)
+    pbl = global_pbl(seed, nx, ny)
+    pbl.precip = precip_patches(nx, ny, seed=seed)
+    rng = np.random.default_rng(seed * 6151)
+    q = np.exp(rng.normal(size=n_sources))
+    counts = np.maximum(1, np.floor(n_particles * q / q.sum())).astype(np.int64)
+    counts[np.argsort(-q)[: n_particles - int(counts.sum())]] += 1
+    margin = 10.0 / deg
+    pos = np.stack([rng.uniform(1.0, nx - 2.0, n_sources), rng.uniform(margin, ny - 1.0 - margin, n_sources),
+                    rng.uniform(10.0, 500.0, n_sources)], axis=1).astype(np.float32)
+    scale = 1.0 / (METRES_PER_DEG * deg)
+    return AdjointCase(met, pbl, counts.astype(np.uint32), q.astype(np.float32), pos,
+                       np.zeros((0, nx * ny), np.float32), np.zeros((0, nx * ny), np.float32),
+                       np.zeros(0, np.float32), 600.0, scale, scale, (seed, 0x57E55))

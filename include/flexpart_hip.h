@@ -216,6 +216,43 @@ int fp_compact(fp_ctx *ctx, uint64_t *active_count, uint32_t *survivor_map);
  * when new_to_old is given. */
 int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *params, uint32_t *new_to_old);
 
+/* ---- source-term injection ------------------------------------------ */
+
+/* GridDomain (coords/mod.rs:72-85): lon/lat of the lower-left corner and the
+ * spacing, in degrees. */
+typedef struct fp_grid_domain {
+    double xlon0, ylat0, dx, dy;
+    uint32_t nx, ny;
+} fp_grid_domain;
+
+/* One release (ReleaseConfig + ReleaseBounds, release/mod.rs:172-199): box
+ * bounds, total mass and particle count, emission window [start_s, end_s]
+ * in simulation seconds, and the release index written to its particles. */
+typedef struct fp_release {
+    double lon_min, lon_max, lat_min, lat_max, z_min, z_max;
+    double mass_kg;
+    uint64_t particle_count;
+    int64_t start_s, end_s;
+    int32_t release_point, class_id;
+} fp_release;
+
+/* 88 bytes */
+
+/* ReleaseManager::inject_for_time (release/mod.rs:90-152) on the device: for
+ * each release in order, the particles its window emits between released[r]
+ * (in/out: emitted so far) and now_s are placed by stratified sampling and
+ * written straight into the current particle set, in emission order, to
+ * slots[0..n_slots) (the free-slot order of ParticleStore::add) or, with
+ * slots == NULL, to first_slot, first_slot + 1, ... (the free slots after
+ * ParticleStore::reset_after_compaction).  *injected receives the count.
+ * A sample outside the grid domain or running out of slots writes the
+ * particles before it, keeps that release's count and returns FP_E_INVALID,
+ * as the reference does.  Blocking. */
+int fp_release_inject(fp_ctx *ctx, const fp_grid_domain *grid, const fp_release *releases,
+                      uint32_t n_releases, uint64_t *released, int64_t now_s,
+                      const uint32_t *slots, uint64_t n_slots, uint64_t first_slot,
+                      uint64_t *injected);
+
 /* ---- temporal weight ------------------------------------------------ */
 
 /* interpolation_alpha (simulation/timeloop.rs:2086-2120): the weight of the
@@ -370,7 +407,8 @@ int fp_ens_last_step_ms(fp_ens *ens, double *ms);
  * particles_per_source[s] particles at its position at t = 0, each with mass
  * q_s / particles_per_source[s].  The backward pass replays the forward's
  * Philox draws (counter-based), restarts from a state checkpoint every
- * checkpoint_every steps, recomputes the steps in between, and propagates
+ * checkpoint_every steps, recomputes the steps in between (replaying the
+ * saved survivor maps when particles are compacted), and propagates
  * the adjoint state back through deposition, the Langevin sub-steps with
  * reflection, the Hanna profiles, advection and the wind interpolation.
  * It is the pathwise (common random numbers) derivative.  No counterpart in
@@ -387,9 +425,18 @@ typedef struct fp_adj_desc {
     uint32_t n_steps;
     uint32_t obs_first_step;          /* observe after steps obs_first_step..n_steps (>= 1) */
     uint32_t checkpoint_every;        /* >= 1 */
-    uint32_t reserved;
+    uint32_t flags;                   /* FP_ADJ_* */
     const uint32_t *particles_per_source; /* n_sources counts (>= 1 each) */
 } fp_adj_desc;
+
+/* compact the particles after every step (the reference's compaction: the
+ * forward saves every step's survivor map and the backward carries the
+ * adjoint back through it) */
+#define FP_ADJ_COMPACT 1u
+/* backward computes the emission gradient only (the adjoint of the masses:
+ * deposition factors, survivor maps and observations; no position or wind
+ * adjoint) */
+#define FP_ADJ_EMISSION_ONLY 2u
 
 int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out);
 int fp_adj_destroy(fp_adj *adj);
@@ -416,8 +463,12 @@ int fp_adj_forward(fp_adj *adj, const fp_step_params *base, const float *emissio
                    double *loss);
 /* Gradient of the loss of the last forward run.  Blocking. */
 int fp_adj_backward(fp_adj *adj, double *grad_emission, double *grad_position, double *grad_coeff);
-/* particle state after the last forward run (n_particles records) */
+/* particle state after the last forward run (the first n slots; after
+ * compaction the active particles come first) */
 int fp_adj_download_state(fp_adj *adj, fp_particle *out, uint64_t n);
+/* active particles at the start of every step of the last forward run and
+ * after its last step (n_steps + 1 values) */
+int fp_adj_active_counts(fp_adj *adj, uint64_t *counts);
 /* device milliseconds of the last forward and backward calls */
 int fp_adj_last_ms(fp_adj *adj, double *forward_ms, double *backward_ms);
 

@@ -835,3 +835,79 @@ void orc_math_eval(int fn, const float *x, const float *y, float *out, uint64_t 
         }
     }
 }
+
+/* ---------------------------------------------------------------- release */
+
+/* target_emitted_count (release/mod.rs:181-197) */
+static uint64_t orc_release_target(const orc_release *r, uint64_t released, int64_t now_s)
+{
+    if (now_s < r->start_s || now_s > r->end_s) return released;
+    if (r->start_s == r->end_s || now_s == r->end_s) return r->particle_count;
+    const uint64_t elapsed = (uint64_t)(now_s - r->start_s);
+    const uint64_t duration = (uint64_t)(r->end_s - r->start_s);
+    return (uint64_t)(((unsigned __int128)r->particle_count * elapsed) / duration);
+}
+
+/* sample_stratified (release/mod.rs:264-283) */
+static void orc_release_sample(const orc_release *r, uint64_t idx, double *lon, double *lat, double *z)
+{
+    double c = ceil(cbrt((double)r->particle_count));
+    if (c < 1.0) c = 1.0;
+    const uint64_t nx = (uint64_t)c, ny = nx;
+    uint64_t nxy = nx * ny;
+    if (nxy < 1) nxy = 1;
+    uint64_t nz = (r->particle_count + nxy - 1) / nxy;
+    if (nz < 1) nz = 1;
+    const uint64_t ix = idx % nx, iy = (idx / nx) % ny, iz = (idx / nxy) % nz;
+    const double fx = ((double)ix + 0.5) / (double)nx;
+    const double fy = ((double)iy + 0.5) / (double)ny;
+    const double fz = ((double)iz + 0.5) / (double)nz;
+    *lon = r->lon_min + fx * (r->lon_max - r->lon_min);
+    *lat = r->lat_min + fy * (r->lat_max - r->lat_min);
+    *z = r->z_min + fz * (r->z_max - r->z_min);
+}
+
+uint64_t orc_release_inject(const orc_grid_domain *grid, const orc_release *rel, uint32_t n_rel,
+                            uint64_t *released, int64_t now_s, int32_t particle_time,
+                            orc_particle *out, uint64_t max_out, int *status)
+{
+    uint64_t w = 0;
+    *status = 0;
+    for (uint32_t k = 0; k < n_rel; ++k) {
+        const orc_release *r = &rel[k];
+        const uint64_t target = orc_release_target(r, released[k], now_s);
+        if (target <= released[k]) continue;
+        const float mass = (float)(r->mass_kg / (double)r->particle_count);
+        for (uint64_t idx = released[k]; idx < target; ++idx) {
+            double lon, lat, z;
+            orc_release_sample(r, idx, &lon, &lat, &z);
+            const double gx = (lon - grid->xlon0) / grid->dx;
+            const double gy = (lat - grid->ylat0) / grid->dy;
+            if (!(gx >= 0.0 && gx < (double)(grid->nx - 1) && gy >= 0.0 && gy < (double)(grid->ny - 1))) {
+                *status = 1;
+                return w;
+            }
+            if (w >= max_out) {
+                *status = 2;
+                return w;
+            }
+            orc_particle p;
+            memset(&p, 0, sizeof p);
+            const double flx = floor(gx), fly = floor(gy);
+            p.cell_x = (int32_t)flx;
+            p.cell_y = (int32_t)fly;
+            p.pos_x = (float)(gx - flx);
+            p.pos_y = (float)(gy - fly);
+            p.pos_z = (float)z;
+            p.flags = FLAG_ACTIVE;
+            p.mass[0] = mass;
+            p.time = particle_time;
+            p.time_mem = particle_time;
+            p.release_point = r->release_point;
+            p.class_id = r->class_id;
+            out[w++] = p;
+        }
+        released[k] = target;
+    }
+    return w;
+}

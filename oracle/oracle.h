@@ -104,6 +104,33 @@ int orc_sort_morton(const orc_particle *p, uint64_t n, uint32_t bits, int explic
 
 void orc_math_eval(int fn, const float *x, const float *y, float *out, uint64_t n);
 
+/* ---- source-term injection (release/mod.rs) --------------------------- */
+
+typedef struct {
+    double xlon0, ylat0, dx, dy; /* GridDomain (coords/mod.rs:72-85) */
+    uint32_t nx, ny;
+} orc_grid_domain;
+
+typedef struct {
+    double lon_min, lon_max, lat_min, lat_max, z_min, z_max; /* ReleaseBounds */
+    double mass_kg;
+    uint64_t particle_count;
+    int64_t start_s, end_s;
+    int32_t release_point, class_id;
+} orc_release;
+
+/* ReleaseManager::inject_for_time (release/mod.rs:90-152): for every release
+ * in order, the particles between released[r] and the target count at now_s
+ * (target_emitted_count, :181-197), placed by sample_stratified (:264-283)
+ * and geo_to_grid / to_relative (coords/mod.rs:109-114, 169-178), written to
+ * out[0..] in emission order.  released[] is advanced per release.  Returns
+ * the number written; *status = 0, 1 (a sample fell outside the grid: the
+ * particles before it are written and that release's count is not advanced)
+ * or 2 (more than max_out particles: max_out are written). */
+uint64_t orc_release_inject(const orc_grid_domain *grid, const orc_release *rel, uint32_t n_rel,
+                            uint64_t *released, int64_t now_s, int32_t particle_time,
+                            orc_particle *out, uint64_t max_out, int *status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -353,3 +353,41 @@ def math_eval(fn: str, x, y=None):
     out = np.zeros_like(x)
     lib().orc_math_eval(MATH_FN[fn], _fp(x), _fp(y), _fp(out), len(x))
     return out
+
+
+# ---------------------------------------------------------------- release
+
+class GridDomain(C.Structure):
+    _fields_ = [("xlon0", C.c_double), ("ylat0", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nx", C.c_uint32), ("ny", C.c_uint32)]
+
+
+class Release(C.Structure):
+    _fields_ = [("lon_min", C.c_double), ("lon_max", C.c_double), ("lat_min", C.c_double),
+                ("lat_max", C.c_double), ("z_min", C.c_double), ("z_max", C.c_double),
+                ("mass_kg", C.c_double), ("particle_count", C.c_uint64),
+                ("start_s", C.c_int64), ("end_s", C.c_int64),
+                ("release_point", C.c_int32), ("class_id", C.c_int32)]
+
+
+def release_inject(grid: dict, releases: list, released, now_s: int, particle_time: int,
+                   max_out: int = 1 << 20):
+    """ReleaseManager::inject_for_time (release/mod.rs:90-152).  grid: dict of
+    GridDomain fields; releases: dicts of Release fields; released: per-release
+    emitted counts (advanced in place).  Returns (particles, status)."""
+    g = GridDomain(**grid)
+    arr = (Release * max(len(releases), 1))()
+    for i, r in enumerate(releases):
+        arr[i] = Release(**r)
+    rel = np.ascontiguousarray(np.asarray(released, np.uint64))
+    out = np.zeros(max_out, PARTICLE_DTYPE)
+    st = C.c_int()
+    L = lib()
+    fn = L.orc_release_inject
+    fn.restype = C.c_uint64
+    fn.argtypes = [C.POINTER(GridDomain), C.POINTER(Release), C.c_uint32, C.POINTER(C.c_uint64), C.c_int64,
+                   C.c_int32, C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]
+    n = fn(C.byref(g), arr, len(releases), rel.ctypes.data_as(C.POINTER(C.c_uint64)), now_s, particle_time,
+           out.ctypes.data, max_out, C.byref(st))
+    released[:] = rel
+    return out[:n].copy(), int(st.value)

@@ -59,10 +59,10 @@ def smooth_case(workloads, case):
     return c
 
 
-def make_adj(fp, c, n_steps=6, obs_first=4, ckpt=4):
+def make_adj(fp, c, n_steps=6, obs_first=4, ckpt=4, compact=False, emission_only=False):
     adj = fp.GpuAdjoint(c.counts, c.met.shape, n_met_levels=len(c.met.levels), n_modes=len(c.coeff),
                         obs_nz=OBS_NZ, obs_dz_m=OBS_DZ, n_steps=n_steps, obs_first_step=obs_first,
-                        checkpoint_every=ckpt)
+                        checkpoint_every=ckpt, compact=compact, emission_only=emission_only)
     adj.upload_met(c.met.levels, c.met.level_heights, c.met.seconds_between_levels)
     adj.upload_basis(c.basis_u, c.basis_v)
     adj.upload_pbl(c.pbl.ustar, c.pbl.wstar, c.pbl.hmix, c.pbl.oli, c.pbl.precip)
@@ -302,6 +302,8 @@ def test_call_order_and_arguments_are_checked(fp, case):
         fp.GpuAdjoint([], c.met.shape)
     with pytest.raises(fp.FpError):
         fp.GpuAdjoint([0, 5], c.met.shape)
+    with pytest.raises(fp.FpError):
+        fp.GpuAdjoint([5], c.met.shape, obs_nz=1)
     adj = fp.GpuAdjoint(c.counts, c.met.shape, n_met_levels=3, n_modes=len(COEFF), obs_nz=OBS_NZ,
                         n_steps=4, obs_first_step=2, checkpoint_every=3)
     with adj:
@@ -319,7 +321,7 @@ def test_call_order_and_arguments_are_checked(fp, case):
         with pytest.raises(fp.FpError):
             adj.set_receptors([[1, 2, 3], [1, 2, 3]])
         with pytest.raises(fp.FpError) as e:
-            adj.forward(cfg_of(fp, c, loss_mode=1), c.emission, c.position, c.coeff)
+            adj.forward(cfg_of(fp, c, dry_mode=2), c.emission, c.position, c.coeff)
         assert e.value.code == fp.FP_E_INVALID
         adj.set_receptors([[1, 2, 3]], [1.0])
         _, loss = adj.forward(cfg_of(fp, c), c.emission, c.position, c.coeff)
@@ -448,3 +450,103 @@ def test_step_vjp_wind_adjoint_matches_directional_difference(fp, vjp_work, work
     adj = float(np.sum(wu * du) + np.sum(wv * dv))
     say(f"  adjoint {adj:.8e}  directional difference {fd:.8e}")
     assert abs(adj - fd) <= 2e-3 * abs(fd)
+
+
+# ---------------------------------------------------------------- compaction
+
+def lossy_cfg(fp, c):
+    """Strong dry and precip-driven wet deposition with the loss rule: the
+    particles of a source die once they keep less than 80 % of the smallest
+    initial particle mass."""
+    m0 = float(np.min(c.emission / c.counts.astype(np.float32)))
+    return cfg_of(fp, c, dry_vd=5e-2, wet_mode=3, wet_a=1e-3, wet_b=0.8, wet_fraction=1.0,
+                  loss_mode=1, loss_min_mass=0.8 * m0)
+
+
+def test_compacting_forward_is_the_oracle_sequence(fp, orc, case):
+    """Step, observe, compact every step: states, active counts and receptor
+    values equal the oracle's step + compaction sequence."""
+    from test_gpu_parity import oracle_params
+    c = case
+    n_steps, obs_first = 6, 4
+    nodes = all_nodes(c)
+    cfg = lossy_cfg(fp, c)
+    say(f"compacting adjoint forward vs oracle: {sum(COUNTS)} particles, {n_steps} steps")
+    with make_adj(fp, c, n_steps, obs_first, compact=True) as adj:
+        adj.set_receptors(nodes)
+        vals, _ = adj.forward(cfg, c.emission, c.position, c.coeff)
+        counts = adj.active_counts()
+        got = adj.download_state(int(counts[-1]))
+    parts = initial_particles(fp, c)
+    lv = perturbed_levels(c, c.coeff)
+    T = int(c.met.seconds_between_levels)
+    table = np.arange(len(nodes))
+    want_vals = np.zeros(len(nodes))
+    counter = tuple(cfg.counter)
+    n_cur = len(parts)
+    want_counts = [n_cur]
+    for k in range(n_steps):
+        t = int(k * c.dt_seconds)
+        b = min(t // T, len(lv) - 2)
+        met = dict(shape=c.met.shape, u0=lv[b][0], v0=lv[b][1], w0=lv[b][2], u1=lv[b + 1][0],
+                   v1=lv[b + 1][1], w1=lv[b + 1][2], level_heights=c.met.level_heights)
+        cfg.alpha = fp.interpolation_alpha(b * T, (b + 1) * T, t)
+        cfg.counter = counter
+        live = parts[:n_cur].copy()
+        counter = tuple(int(v) for v in orc.step(live, met, pbl_dict(c.pbl), oracle_params(c, cfg)))
+        if k + 1 >= obs_first:
+            cic_values(c, live, table, want_vals)
+        n_cur, out, _ = orc.compact(live)
+        parts[: len(out)] = out
+        want_counts.append(n_cur)
+    assert list(counts) == want_counts
+    assert want_counts[-1] < want_counts[0], "the case must lose particles"
+    assert got.tobytes() == parts[: want_counts[-1]].tobytes()
+    np.testing.assert_allclose(vals, want_vals, rtol=1e-9, atol=1e-12)
+
+
+def test_identity_compaction_leaves_the_gradients_unchanged(fp, case, receptors):
+    """Without losses every compaction is the identity: the saved-map replay
+    and the adjoint's un-compaction give the gradients of the run without
+    compaction; the emission-only backward gives its emission gradient."""
+    c = case
+    nodes, obs = receptors
+    say("identity compaction: gradients with and without the map path")
+    grads = []
+    for compact, emission_only in ((False, False), (True, False), (True, True)):
+        with make_adj(fp, c, compact=compact, emission_only=emission_only) as adj:
+            adj.set_receptors(nodes, obs)
+            adj.forward(cfg_of(fp, c), c.emission, c.position, c.coeff)
+            assert list(adj.active_counts()) == [sum(COUNTS)] * 7
+            grads.append(adj.backward())
+    (q0, r0, k0), (q1, r1, k1), (q2, r2, k2) = grads
+    np.testing.assert_allclose(q1, q0, rtol=1e-6)
+    np.testing.assert_allclose(r1, r0, rtol=1e-5, atol=1e-6 * np.max(np.abs(r0)))
+    np.testing.assert_allclose(k1, k0, rtol=1e-5)
+    np.testing.assert_allclose(q2, q0, rtol=1e-5)
+    assert not np.any(r2) and not np.any(k2)
+
+
+def test_emission_only_backward_matches_the_full_adjoint_through_losses(fp, case):
+    """With particles lost and compacted every step, the emission-only
+    backward (survival factors through the saved survivor maps) and the full
+    adjoint give the same emission gradient."""
+    c = case
+    cfg = lossy_cfg(fp, c)
+    nodes = all_nodes(c)
+    with make_adj(fp, c, compact=True) as adj:
+        adj.set_receptors(nodes)
+        base, _ = adj.forward(cfg, c.emission, c.position, c.coeff)
+    hit = np.nonzero(base > 0)[0]
+    obs = base[hit] * np.random.default_rng(41).uniform(0.5, 1.5, len(hit))
+    out = []
+    for emission_only in (False, True):
+        with make_adj(fp, c, compact=True, emission_only=emission_only) as adj:
+            adj.set_receptors(nodes[hit], obs)
+            adj.forward(cfg, c.emission, c.position, c.coeff)
+            counts = adj.active_counts()
+            out.append(adj.backward())
+    say(f"  active counts {list(counts)}; emission gradients {out[0][0]} vs {out[1][0]}")
+    assert counts[-1] < counts[0]
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-5, atol=1e-7 * np.max(np.abs(out[0][0])))
+    assert np.all(np.isfinite(out[0][1])) and np.all(np.isfinite(out[0][2]))
