@@ -94,7 +94,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
     ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--loss-fraction", type=float, default=0.5,
+    ap.add_argument("--loss-fraction", type=float, default=0.3,
                     help="stress workload: a particle is lost once its mass falls below this "
                          "fraction of the common initial particle mass")
     ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress"), default="global_10m",
