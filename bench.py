@@ -55,30 +55,44 @@ def _traffic_bytes():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(w, threads: int, sample_every: int = 5, steps: int = 4):
-    """The oracle on every ``sample_every``-th particle for ``steps`` steps."""
+def cpu_baseline(w, threads: int, sample_every: int = 5, steps: int = 10):
+    """The oracle on every ``sample_every``-th particle of the workload: one
+    warm-up step, then ``steps`` timed steps (median step time), with OpenMP
+    over ``threads`` host cores; and single-threaded on a tenth of that
+    sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle
 
-    parts = w.particles[::sample_every].copy()
     u0, v0, w0 = w.met.levels[0]
     u1, v1, w1 = w.met.levels[1]
     met = dict(shape=w.met.shape, u0=u0, v0=v0, w0=w0, u1=u1, v1=v1, w1=w1,
                level_heights=w.met.level_heights)
     pbl = dict(shape=w.pbl.shape, ustar=w.pbl.ustar, wstar=w.pbl.wstar, hmix=w.pbl.hmix,
                oli=w.pbl.oli)
-    counter = (0, 0, 0, 0)
-    t0 = time.perf_counter()
-    for k in range(steps):
-        counter = oracle.step(parts, met, pbl, dict(
-            dt_seconds=w.dt_seconds, alpha=k * w.dt_seconds / 21600.0, x_scale=w.x_scale,
-            y_scale=w.y_scale, key=w.key, counter=counter, dry_mode=1, dry_vd=2e-3,
-            n_threads=threads))
-    dt = time.perf_counter() - t0
-    return {"value": len(parts) * steps / dt, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{len(parts)} particles (every {sample_every}th of the 10M workload) x "
-                      f"{steps} steps, C oracle with OpenMP, {dt:.2f} s wall"}
+
+    def rate(parts, n_threads):
+        counter = (0, 0, 0, 0)
+        times = []
+        for k in range(steps + 1):
+            t0 = time.perf_counter()
+            counter = oracle.step(parts, met, pbl, dict(
+                dt_seconds=w.dt_seconds, alpha=k * w.dt_seconds / 21600.0, x_scale=w.x_scale,
+                y_scale=w.y_scale, key=w.key, counter=counter, dry_mode=1, dry_vd=2e-3,
+                n_threads=n_threads))
+            if k:
+                times.append(time.perf_counter() - t0)
+        return len(parts) / float(np.median(times)), float(np.sum(times))
+
+    parts = w.particles[::sample_every].copy()
+    value, wall = rate(parts, threads)
+    single, wall1 = rate(w.particles[::sample_every * 10].copy(), 1)
+    return {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(parts)} particles (every {sample_every}th of the 10M workload), 1 warm-up + "
+                      f"{steps} timed steps, median step, C oracle with OpenMP, {wall:.2f} s wall",
+            "single_thread_value": single,
+            "single_thread_sample": f"{len(parts) // 10} particles x {steps} steps, 1 thread, "
+                                    f"{wall1:.2f} s wall"}
 
 
 def main(argv=None) -> int:
