@@ -223,9 +223,7 @@ __global__ void __launch_bounds__(256) adj_step_kernel(AdjArgs g)
         const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
         const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
         const uint32_t pidx = pi * a.ny + pj;
-        const float4 pb = a.pbl[pidx];
-        const float ol = fabsf(pb.w) < 1.0e-5f ? __builtin_inff() : 1.0f / pb.w;
-        const TurbT<Dz> t = hanna_full(pb.x, pb.y, ol, pb.z, Dz{z1, 1.0f}, a.log_1em3);
+        const TurbT<Dz> t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], Dz{z1, 1.0f}, a.log_1em3);
 
         const U4 ctr{a.c0, a.c1, a.c2, a.c3};
         const uint64_t cbase = i * a.blocks_per_particle;
@@ -956,7 +954,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     }
     C_HIP(amalloc(&e->pert_u, e->nxy));
     C_HIP(amalloc(&e->pert_v, e->nxy));
-    C_HIP(amalloc(&e->pbl, e->nxy));
+    C_HIP(amalloc(&e->pbl, e->nxy * 2u));
     C_HIP(amalloc(&e->precip, e->nxy));
     C_HIP(amalloc(&e->pbl_stage, e->nxy * 4u));
     C_HIP(amalloc(&e->src_of, total));

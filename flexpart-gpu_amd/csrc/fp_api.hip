@@ -280,7 +280,7 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
     CREATE_HIP(dalloc(&ctx->met, npts * 2));
     CREATE_HIP(dalloc(&ctx->level_heights, (uint64_t)d.met_nz));
     const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
-    CREATE_HIP(dalloc(&ctx->pbl, np));
+    CREATE_HIP(dalloc(&ctx->pbl, np * 2));
     CREATE_HIP(dalloc(&ctx->precip, np));
     if (ncell) {
         CREATE_HIP(dalloc(&ctx->grid_counts, ncell));

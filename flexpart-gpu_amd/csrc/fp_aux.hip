@@ -1,13 +1,10 @@
 // fp_aux.hip - data movement kernels around the step: met/PBL packing,
 // AoS <-> SoA transposes, stable stream compaction and the Philox
 // known-answer kernel.
-#include "fp_internal.hpp"
-#include "fp_math.h"
+#include "fp_physics.hpp"
 
 namespace fp {
 namespace {
-
-constexpr uint32_t FLAG_ACTIVE = 1u;
 
 // 32-byte met record per grid point: {u0, v0, w0, u1, v1, w1, 0, 0}.  One
 // record holds both time levels of all three components, so a trilinear
@@ -22,11 +19,15 @@ __global__ void pack_met_kernel(const float *u0, const float *v0, const float *w
     met[2u * i + 1u] = make_float4(v1[i], w1[i], 0.0f, 0.0f);
 }
 
+// Two 16-byte records per PBL cell: {u*, w*, hmix, 1/L} as uploaded and the
+// cell's height-independent Hanna terms (pbl_cell_terms_dev).
 __global__ void pack_pbl_kernel(const float *stage, float4 *pbl, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    pbl[i] = make_float4(stage[i], stage[n + i], stage[2u * n + i], stage[3u * n + i]);
+    const float4 pb = make_float4(stage[i], stage[n + i], stage[2u * n + i], stage[3u * n + i]);
+    pbl[2u * i] = pb;
+    pbl[2u * i + 1u] = pbl_cell_terms_dev(pb);
 }
 
 // AoS (96 B records) -> SoA.  slots == nullptr writes record r to slot r.

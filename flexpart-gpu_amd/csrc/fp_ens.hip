@@ -212,7 +212,7 @@ int fp_ens_create(int device_ordinal, const fp_ens_desc *desc, fp_ens **out)
     C_HIP(dmalloc(&e->met, npts * 2 * d.n_members));
     C_HIP(dmalloc(&e->level_heights, (uint64_t)d.met_nz));
     const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
-    C_HIP(dmalloc(&e->pbl, np));
+    C_HIP(dmalloc(&e->pbl, np * 2));
     C_HIP(dmalloc(&e->precip, np));
     C_HIP(dmalloc(&e->pbl_stage, np * 4));
     if (e->ncell()) {

@@ -65,7 +65,8 @@ struct StepArgs {
     const float *level_heights;  // nz floats (device)
     uint32_t has_heights;
     float zmax;
-    // PBL: packed {ustar, wstar, hmix, oli} per (i, j); precip optional
+    // PBL: two 16-byte records per cell (i, j): {ustar, wstar, hmix, oli} and
+    // {L, unstable sigma_u, regime, 0} (pbl_cell_terms_dev); precip optional
     const float4 *pbl;
     const float *precip;
     // forcing
@@ -151,7 +152,7 @@ struct fp_ctx {
     float zmax = 0.0f;
     float *met_stage = nullptr;    // 6 * npoints floats for host uploads
 
-    // PBL
+    // PBL (2 records per cell)
     float4 *pbl = nullptr;
     float *precip = nullptr;
     bool pbl_ready = false;

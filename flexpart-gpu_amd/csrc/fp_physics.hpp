@@ -323,18 +323,32 @@ __device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, do
     if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
 }
 
-// compute_hanna (langevin_fused.wgsl:224-293)
+// The per-cell part of compute_hanna, evaluated once per PBL upload: the
+// Obukhov length (obukhov_length_from_inverse, langevin_fused.wgsl:213-218),
+// the regime (0 neutral, 1 unstable, 2 stable) and the unstable regime's
+// height-independent sigma_u.  Stored beside {u*, w*, hmix, 1/L} as the
+// cell's second 16-byte record.
+__device__ __forceinline__ float4 pbl_cell_terms_dev(float4 pb)
+{
+    const float ust = fmaxf(pb.x, USTAR_MIN), h = fmaxf(pb.z, H_MIN);
+    const float ol = fabsf(pb.w) < 1.0e-5f ? __builtin_inff() : 1.0f / pb.w;
+    const bool neu = neutral_regime(ol, h), uns = !neu && ol < 0.0f;
+    const float sigu = uns ? SIGMA_FLOOR + ust * fpm_powf(12.0f - 0.5f * h / ol, 1.0f / 3.0f) : 0.0f;
+    return make_float4(ol, sigu, __uint_as_float(neu ? 0u : (uns ? 1u : 2u)), 0.0f);
+}
+
+// compute_hanna (langevin_fused.wgsl:224-293) from the cell's two records
 template <class T>
-__device__ __forceinline__ TurbT<T> hanna_full(float ust_in, float wst_in, float ol, float h_in, T z_in,
-                                               double log_1em3)
+__device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, double log_1em3)
 {
     TurbT<T> t;
-    t.ust = fmaxf(ust_in, USTAR_MIN);
-    t.wst = fmaxf(wst_in, 0.0f);
-    t.h = fmaxf(h_in, H_MIN);
-    t.ol = ol;
-    t.neu = neutral_regime(ol, t.h);
-    t.uns = !t.neu && ol < 0.0f;
+    t.ust = fmaxf(pb.x, USTAR_MIN);
+    t.wst = fmaxf(pb.y, 0.0f);
+    t.h = fmaxf(pb.z, H_MIN);
+    t.ol = pc.x;
+    const uint32_t regime = __float_as_uint(pc.z);
+    t.neu = regime == 0u;
+    t.uns = regime == 1u;
     const float ust = t.ust, h = t.h;
     const T z = tclamp(z_in, 0.0f, h);
     const T zeta = tclamp(z / h, 0.0f, 1.0f);
@@ -344,7 +358,7 @@ __device__ __forceinline__ TurbT<T> hanna_full(float ust_in, float wst_in, float
         sigu = SIGMA_FLOOR + 2.0f * ust * texp(-3.0e-4f * v.d0);
         sigv = v.sigw;
     } else if (t.uns) {
-        sigu = kon<T>(SIGMA_FLOOR + ust * fpm_powf(12.0f - 0.5f * h / ol, 1.0f / 3.0f));
+        sigu = kon<T>(pc.y);
         sigv = sigu;
     } else {
         sigu = SIGMA_FLOOR + 2.0f * ust * (1.0f - zeta);

@@ -72,9 +72,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
             const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
             const uint32_t pidx = pi * a.ny + pj;
-            const float4 pb = a.pbl[pidx];
-            const float ol = fabsf(pb.w) < 1.0e-5f ? __builtin_inff() : 1.0f / pb.w;
-            const Turb t = hanna_full(pb.x, pb.y, ol, pb.z, pz, a.log_1em3);
+            const Turb t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], pz, a.log_1em3);
 
             // ---- Philox blocks per slot (langevin_fused.wgsl:430-451)
             const uint64_t cbase = i * a.blocks_per_particle;
