@@ -479,29 +479,125 @@ __device__ __forceinline__ uint32_t quantize_mass(float m_in, float scale)
 
 }  // namespace
 
-// Wave-level pre-reduction of same-cell contributions followed by one pair of
-// global atomics per distinct cell in the wave.  Every lane of the wave must
-// call this (lanes without a contribution pass valid = false).  Integer sums
-// wrap modulo 2^32 exactly as the reference's atomicAdd does, so the result
-// does not depend on the order of reduction.
-__device__ __forceinline__ void grid_reduce_wave(const GridArgs &g, bool valid, uint32_t cell, uint32_t q)
+// Sum of v over the wave, complete in lane 63 (the other lanes hold partial
+// prefix sums).  DPP row shifts build the 16-lane row sums and the two row
+// broadcasts carry them across rows: six VALU operations with no LDS traffic.
+__device__ __forceinline__ uint32_t wave_sum_to_lane63(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return v;
+}
+
+// Block-private concentration grid in LDS: an open-addressing hash table of
+// (cell, count, mass) that absorbs a block's contributions; the block adds
+// each occupied slot to the global grid once at its end.  Point releases put
+// most particles of a launch into a handful of output cells, and without the
+// table every wave would issue atomics to those same addresses, which the
+// memory system serialises.  A cell that finds no free slot within
+// kGridProbes probes goes straight to the global grid.  All sums are integer
+// sums modulo 2^32, so neither the table nor the order of the adds changes
+// the result.
+constexpr uint32_t kGridSlotBits = 9;
+constexpr uint32_t kGridSlots = 1u << kGridSlotBits;
+constexpr uint32_t kGridProbes = 8;
+constexpr uint32_t kGridEmpty = 0xffffffffu;  // no cell index reaches 2^32 - 1
+
+struct GridLds {
+    uint32_t key[kGridSlots];
+    uint32_t cnt[kGridSlots];
+    uint32_t mass[kGridSlots];
+};
+
+// Empties the table; the caller's barrier must follow before first use.
+template <uint32_t BLOCK>
+__device__ __forceinline__ void grid_lds_clear(GridLds &t)
+{
+    for (uint32_t k = threadIdx.x; k < kGridSlots; k += BLOCK) {
+        t.key[k] = kGridEmpty;
+        t.cnt[k] = 0u;
+        t.mass[k] = 0u;
+    }
+}
+
+// Workgroup-scope atomics for the table: besides stating the scope, they
+// keep the compiler from merging a table add and a global-grid add into one
+// flat atomic with a selected address.
+__device__ __forceinline__ void lds_add(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_cas(uint32_t *p, uint32_t expected, uint32_t desired)
+{
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expected;
+}
+
+__device__ __forceinline__ void grid_lds_add(GridLds &t, const GridArgs &g, uint32_t cell, uint32_t c, uint32_t m)
+{
+    uint32_t h = (cell * 0x9E3779B1u) >> (32u - kGridSlotBits);
+    for (uint32_t p = 0; p < kGridProbes; ++p) {
+        const uint32_t prev = lds_cas(&t.key[h], kGridEmpty, cell);
+        if (prev == kGridEmpty || prev == cell) {
+            lds_add(&t.cnt[h], c);
+            lds_add(&t.mass[h], m);
+            return;
+        }
+        h = (h + 1u) & (kGridSlots - 1u);
+    }
+    atomicAdd(&g.counts[cell], c);
+    if (m) atomicAdd(&g.mass_fp[cell], m);
+}
+
+// Adds the table to the global grid; every thread of the block must call it.
+template <uint32_t BLOCK>
+__device__ __forceinline__ void grid_lds_flush(GridLds &t, const GridArgs &g)
+{
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kGridSlots; k += BLOCK) {
+        const uint32_t cell = t.key[k];
+        if (cell == kGridEmpty) continue;
+        atomicAdd(&g.counts[cell], t.cnt[k]);
+        const uint32_t m = t.mass[k];
+        if (m) atomicAdd(&g.mass_fp[cell], m);
+    }
+}
+
+// Wave-level pre-reduction of same-cell contributions into the block's
+// table.  Every lane of the wave must call this (lanes without a
+// contribution pass valid = false).  Each pass takes the lowest pending
+// lane's cell (v_readlane with a wave-uniform index), counts the lanes with
+// that cell and sums their masses with a DPP reduction; the totals of pass j
+// are kept in lane j, and the distinct cells of the wave then enter the
+// table in parallel, one per lane.
+__device__ __forceinline__ void grid_reduce_wave(GridLds &t, const GridArgs &g, bool valid, uint32_t cell,
+                                                 uint32_t q)
 {
     uint64_t pending = __ballot(valid);
     const uint32_t lane = __lane_id();
+    uint32_t my_cell = 0u, my_cnt = 0u, my_mass = 0u;
+    uint32_t j = 0;
     while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
-        const uint32_t lc = __shfl(cell, (int)leader);
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane((int)cell, (int)leader);
         const bool mine = valid && cell == lc;
         const uint64_t same = __ballot(mine);
-        uint32_t v = mine ? q : 0u;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == leader) {
-            atomicAdd(&g.counts[lc], (uint32_t)__popcll(same));
-            atomicAdd(&g.mass_fp[lc], v);
+        const uint32_t v = wave_sum_to_lane63(mine ? q : 0u);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(kWave - 1u));
+        if (lane == j) {
+            my_cell = lc;
+            my_cnt = (uint32_t)__popcll(same);
+            my_mass = tot;
         }
+        ++j;
         pending &= ~same;
     }
+    if (lane < j) grid_lds_add(t, g, my_cell, my_cnt, my_mass);
 }
 
 // XCD-aware chunking: the hardware deals blocks round-robin over the 8

@@ -28,27 +28,49 @@ namespace fp {
 // r < a.ppt, temporal weight alpha and Philox base counter ctr.
 template <uint32_t BLOCK, int NSUB>
 __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, float alpha, U4 ctr,
-                                           const float *lh, const float *oh)
+                                           const float *lh, const float *oh, GridLds &gt)
 {
     const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
     const Soa s = a.s;
+    // Philox blocks per particle: 2 when the vertical loop takes more than two
+    // normals (known at compile time for NSUB >= 0, so both blocks interleave)
+    const bool two_blocks = NSUB >= 0 ? NSUB > 2 : a.blocks_per_particle > 1u;
     for (uint32_t r = 0; r < a.ppt; ++r) {
         const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
-        bool live = i < a.n;
+        const bool inb = i < a.n;
+        // The slot's whole hot state is requested at once (44 B, coalesced);
+        // slots whose flags say inactive are not used further.
         uint32_t flags = 0;
-        if (live) flags = s.u(F_FLAGS)[i];
-        live = live && (flags & FLAG_ACTIVE);
-        bool grid_valid = false;
-        uint32_t grid_c = 0, grid_q = 0;
-        if (live) {
-            int32_t cx = s.i(F_CELL_X)[i], cy = s.i(F_CELL_Y)[i];
-            float px = s.f(F_POS_X)[i], py = s.f(F_POS_Y)[i], pz = s.f(F_POS_Z)[i];
-            float tu = s.f(F_TURB_U)[i], tv = s.f(F_TURB_V)[i], tw = s.f(F_TURB_W)[i];
-            int32_t cbt = s.i(F_CBT)[i];
-            float mass[FP_MAX_SPECIES];
+        int32_t cx = 0, cy = 0, cbt = 0;
+        float px = 0.0f, py = 0.0f, pz = 0.0f, tu = 0.0f, tv = 0.0f, tw = 0.0f;
+        float mass[FP_MAX_SPECIES];
+        if (inb) {
+            flags = s.u(F_FLAGS)[i];
+            cx = s.i(F_CELL_X)[i]; cy = s.i(F_CELL_Y)[i];
+            px = s.f(F_POS_X)[i]; py = s.f(F_POS_Y)[i]; pz = s.f(F_POS_Z)[i];
+            tu = s.f(F_TURB_U)[i]; tv = s.f(F_TURB_V)[i]; tw = s.f(F_TURB_W)[i];
+            cbt = s.i(F_CBT)[i];
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
                 if (q < a.n_species) mass[q] = s.f(F_MASS0 + q)[i];
+        }
+
+        // ---- Philox blocks per slot (langevin_fused.wgsl:430-451) and the
+        // Box-Muller normals: they depend on the slot index only, so they are
+        // computed while the state loads are in flight.
+        const uint64_t cbase = i * a.blocks_per_particle;
+        const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
+        U4 b1 = U4{0u, 0u, 0u, 0u};
+        if (two_blocks) b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
+        float eu, ev, ew0, ew1, ew2, ew3;
+        gauss_pair(b0.x, b0.y, eu, ev);
+        gauss_pair(b0.z, b0.w, ew0, ew1);
+        gauss_pair(b1.x, b1.y, ew2, ew3);
+
+        const bool live = inb && (flags & FLAG_ACTIVE);
+        bool grid_valid = false;
+        uint32_t grid_c = 0, grid_q = 0;
+        if (live) {
 
             // ---- advection (advection_dual_wind.wgsl:254-303)
             const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
@@ -73,17 +95,6 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
             const uint32_t pidx = pi * a.ny + pj;
             const Turb t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], pz, a.log_1em3);
-
-            // ---- Philox blocks per slot (langevin_fused.wgsl:430-451)
-            const uint64_t cbase = i * a.blocks_per_particle;
-            const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
-            U4 b1 = U4{0u, 0u, 0u, 0u};
-            if (a.blocks_per_particle > 1u)
-                b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
-            float eu, ev, ew0, ew1, ew2, ew3;
-            gauss_pair(b0.x, b0.y, eu, ev);
-            gauss_pair(b0.z, b0.w, ew0, ew1);
-            gauss_pair(b1.x, b1.y, ew2, ew3);
 
             // ---- horizontal OU over the full step
             tu = ou_horizontal(tu, t.sigu, t.tlu, a.dt, eu);
@@ -165,7 +176,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
                 grid_q = quantize_mass(mass[a.grid.species], a.grid.mass_scale);
             }
         }
-        if (a.grid_fused) grid_reduce_wave(a.grid, grid_valid, grid_c, grid_q);
+        if (a.grid_fused) grid_reduce_wave(gt, a.grid, grid_valid, grid_c, grid_q);
     }
 }
 
@@ -187,9 +198,12 @@ __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
 {
     __shared__ float lh[kMaxLevels];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
+    __shared__ GridLds gt;
+    if (a.grid_fused) grid_lds_clear<BLOCK>(gt);
     load_tables<BLOCK>(a, lh, oh);
     const uint64_t first = (uint64_t)xcd_chunk(blockIdx.x, gridDim.x) * BLOCK * a.ppt;
-    step_chunk<BLOCK, NSUB>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh);
+    step_chunk<BLOCK, NSUB>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
+    if (a.grid_fused) grid_lds_flush<BLOCK>(gt, a.grid);
 }
 
 // ---------------------------------------------------------------- ensembles
@@ -218,30 +232,40 @@ __global__ void __launch_bounds__(BLOCK, 3) ens_step_kernel(const EnsMember *mem
 {
     __shared__ float lh[kMaxLevels];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
-    load_tables<BLOCK>(members[0].a, lh, oh);  // one vertical grid for all members
+    __shared__ GridLds gt;
     const EnsBlock eb = blocks[xcd_chunk(blockIdx.x, gridDim.x)];
     const EnsMember &m = members[eb.member];
+    if (m.a.grid_fused) grid_lds_clear<BLOCK>(gt);
+    load_tables<BLOCK>(members[0].a, lh, oh);  // one vertical grid for all members
     const U4 ctr = ctr_add(m.base_ctr[0], m.base_ctr[1], m.base_ctr[2], m.base_ctr[3],
                            (uint64_t)clock->step * m.ctr_per_step);
-    step_chunk<BLOCK, NSUB>(m.a, eb.first, clock->alpha, ctr, lh, oh);
+    step_chunk<BLOCK, NSUB>(m.a, eb.first, clock->alpha, ctr, lh, oh, gt);
+    if (m.a.grid_fused) grid_lds_flush<BLOCK>(gt, m.a.grid);
 }
 
-template <uint32_t BLOCK>
+// Standalone gridding: ROWS rows of BLOCK particles per block share one LDS table.
+template <uint32_t BLOCK, uint32_t ROWS>
 __global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs g)
 {
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
+    __shared__ GridLds gt;
+    grid_lds_clear<BLOCK>(gt);
     if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = g.outheights[threadIdx.x];
     __syncthreads();
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    bool valid = false;
-    uint32_t cell = 0, q = 0;
-    if (i < n && (s.u(F_FLAGS)[i] & FLAG_ACTIVE)) {
-        valid = true;
-        cell = grid_cell(g, oh, s.i(F_CELL_X)[i], s.i(F_CELL_Y)[i], s.f(F_POS_X)[i], s.f(F_POS_Y)[i],
-                         s.f(F_POS_Z)[i]);
-        q = quantize_mass(s.f(F_MASS0 + g.species)[i], g.mass_scale);
+    const uint64_t first = (uint64_t)blockIdx.x * BLOCK * ROWS;
+    for (uint32_t r = 0; r < ROWS; ++r) {
+        const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
+        bool valid = false;
+        uint32_t cell = 0, q = 0;
+        if (i < n && (s.u(F_FLAGS)[i] & FLAG_ACTIVE)) {
+            valid = true;
+            cell = grid_cell(g, oh, s.i(F_CELL_X)[i], s.i(F_CELL_Y)[i], s.f(F_POS_X)[i], s.f(F_POS_Y)[i],
+                             s.f(F_POS_Z)[i]);
+            q = quantize_mass(s.f(F_MASS0 + g.species)[i], g.mass_scale);
+        }
+        grid_reduce_wave(gt, g, valid, cell, q);
     }
-    grid_reduce_wave(g, valid, cell, q);
+    grid_lds_flush<BLOCK>(gt, g);
 }
 
 template <uint32_t BLOCK>
@@ -294,8 +318,9 @@ hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, Ens
 hipError_t launch_grid(const Soa &s, uint32_t n, const GridArgs &g, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = (n + 255u) / 256u;
-    grid_kernel<256><<<grid, 256, 0, st>>>(s, n, g);
+    constexpr uint32_t kRows = 8;
+    const uint32_t grid = (uint32_t)(((uint64_t)n + 256u * kRows - 1u) / (256u * kRows));
+    grid_kernel<256, kRows><<<grid, 256, 0, st>>>(s, n, g);
     return hipGetLastError();
 }
 

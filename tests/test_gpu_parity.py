@@ -207,6 +207,45 @@ def test_grid_level_index_and_saturating_quantisation(fp, orc):
     assert np.array_equal(counts, oc) and np.array_equal(mass, om)
 
 
+@pytest.mark.parametrize("block,ppt", [(256, 1), (256, 8), (1024, 2)])
+def test_grid_block_tables_with_hot_cells_and_overflow_match_oracle(fp, orc, small, block, ppt):
+    """Half the particles sit in three hot cells (every block's table absorbs
+    them), the other half are spread over the whole output grid, so a block
+    meets far more distinct cells than its 512-slot table holds and the
+    overflow goes to the global grid directly; fused and standalone paths."""
+    say(f"grid parity: hot cells + table overflow, block={block} ppt={ppt}")
+    from flexpart_gpu_amd.workloads import make_particles
+    rng = np.random.default_rng(31)
+    n = len(small.particles)
+    nx, ny, _ = small.met.shape
+    hot = rng.integers(0, 3, n)
+    spread = rng.uniform(size=n) < 0.5
+    x = np.where(spread, rng.uniform(0, nx - 1, n), 10.2 + 7.0 * hot + rng.uniform(0, 0.05, n))
+    y = np.where(spread, rng.uniform(0, ny - 1, n), 12.3 + 5.0 * hot + rng.uniform(0, 0.05, n))
+    z = np.where(spread, rng.uniform(0, 12_000.0, n), 50.0 + rng.uniform(0, 5.0, n))
+    p = make_particles(n, x, y, z, np.exp(rng.normal(size=n)))
+    p["flags"][rng.uniform(size=n) < 0.05] = 0
+    for fused in (False, True):
+        want = p.copy()
+        with make_ctx(fp, small) as ctx:
+            ctx.set_launch(block, ppt)
+            ctx.upload_store(p)
+            ctx.configure_grid(small.outheights, small.refine, 0, small.mass_scale)
+            ctx.clear_grid()
+            cfg = base_cfg(fp, small, grid_fused=fused)
+            if fused:
+                ctx.step(cfg)
+                orc.step(want, met_dict(small.met), pbl_dict(small.pbl), oracle_params(small, cfg))
+            else:
+                ctx.accumulate_grid()
+            counts, mass = ctx.download_grid()
+        oc, om = orc.grid_accumulate(want, small.out_shape, small.outheights, small.refine, 0,
+                                     small.mass_scale)
+        assert oc.max() > 1000 and np.count_nonzero(oc) > 2000
+        assert np.array_equal(counts, oc), f"counts differ (fused={fused})"
+        assert np.array_equal(mass, om), f"mass differs (fused={fused})"
+
+
 @pytest.mark.parametrize("pattern", ["every_third", "all_active", "all_inactive", "single",
                                      "random"])
 def test_compaction_matches_oracle(fp, orc, small, pattern):
