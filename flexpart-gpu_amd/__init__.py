@@ -141,6 +141,26 @@ class AdjDesc(C.Structure):
 ADJ_COMPACT, ADJ_EMISSION_ONLY = 1, 2
 
 
+class PblOptions(C.Structure):
+    """fp_pbl_options: PblComputationOptions (io/pbl_params.rs:36-71); the
+    defaults are the reference's."""
+    _fields_ = [("roughness_length_m", C.c_float), ("wind_reference_height_m", C.c_float),
+                ("heat_flux_neutral_threshold_w_m2", C.c_float), ("fallback_mixing_height_m", C.c_float),
+                ("hmix_min_m", C.c_float), ("hmix_max_m", C.c_float)]
+
+    def __init__(self, **kw):
+        d = dict(roughness_length_m=0.1, wind_reference_height_m=10.0, heat_flux_neutral_threshold_w_m2=1.0,
+                 fallback_mixing_height_m=800.0, hmix_min_m=100.0, hmix_max_m=4500.0)
+        d.update(kw)
+        super().__init__(**d)
+
+
+# the ten surface fields of fp_surface_upload, in FP_SURF_* order
+SURFACE_FIELDS = ("surface_pressure_pa", "temperature_2m_k", "u10_ms", "v10_ms", "surface_stress_n_m2",
+                  "sensible_heat_flux_w_m2", "solar_radiation_w_m2", "mixing_height_m", "friction_velocity_ms",
+                  "inv_obukhov_length_per_m")
+
+
 _f = C.POINTER(C.c_float)
 _d = C.POINTER(C.c_double)
 _u32 = C.POINTER(C.c_uint32)
@@ -164,6 +184,10 @@ ABI = {
     "fp_met_upload_bracket_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
     "fp_pbl_upload": (C.c_int, [_vp, _f, _f, _f, _f, _f]),
     "fp_forcing_upload": (C.c_int, [_vp, _f, _f, _f]),
+    "fp_surface_upload": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int64]),
+    "fp_surface_upload_dev": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int64]),
+    "fp_pbl_from_surface": (C.c_int, [_vp, C.POINTER(PblOptions), C.c_int64, C.c_int, _vp]),
+    "fp_pbl_download": (C.c_int, [_vp, _f, _f, _f, _f]),
     "fp_step": (C.c_int, [_vp, C.POINTER(StepParams), _u32]),
     "fp_step_vjp": (C.c_int, [_vp, C.POINTER(StepParams), _f, _f, _d, _d]),
     "fp_grid_clear": (C.c_int, [_vp]),
@@ -384,6 +408,49 @@ class GpuTimestep:
         arrs = [_f32(a, n) for a in (ustar, wstar, hmix, oli)]
         pr = None if precip is None else _f32(precip, n)
         self._chk(self._lib.fp_pbl_upload(self._h, *[_fp(a) for a in arrs], _fp(pr)))
+
+    def upload_surface(self, level: int, fields, time_s: int):
+        """Surface fields of bracket end `level` (0: t0, 1: t1) valid at
+        time_s (fp_surface_upload).  fields: mapping with the SURFACE_FIELDS
+        names, or a sequence in that order, of met_nx*met_ny arrays; torch
+        CUDA tensors are copied device to device."""
+        seq = [fields[k] for k in SURFACE_FIELDS] if hasattr(fields, "keys") else list(fields)
+        if len(seq) != len(SURFACE_FIELDS):
+            raise ValueError(f"{len(SURFACE_FIELDS)} surface fields required")
+        n = self.met_shape[0] * self.met_shape[1]
+        ptrs = (_vp * len(seq))()
+        if all(hasattr(a, "data_ptr") and getattr(a, "is_cuda", False) for a in seq):
+            for k, a in enumerate(seq):
+                if a.numel() != n or not a.is_contiguous() or str(a.dtype) != "torch.float32":
+                    raise ValueError("surface fields must be contiguous float32 tensors of met_nx*met_ny")
+                ptrs[k] = a.data_ptr()
+            self._chk(self._lib.fp_surface_upload_dev(self._h, level, ptrs, int(time_s)))
+            return
+        keep = [_f32(a, n) for a in seq]
+        for k, a in enumerate(keep):
+            ptrs[k] = a.ctypes.data
+        self._chk(self._lib.fp_surface_upload(self._h, level, ptrs, int(time_s)))
+
+    def pbl_from_surface(self, options: "PblOptions | None" = None, now_s: int = 0, strict: bool = False,
+                         passthrough=None):
+        """PBL state from the resident surface bracket at now_s
+        (fp_pbl_from_surface).  passthrough: optional contiguous float32 CUDA
+        tensor of 3*met_nx*met_ny that receives sanitized sshf, ssr, stress."""
+        o = PblOptions() if options is None else options
+        pp = None
+        if passthrough is not None:
+            n = self.met_shape[0] * self.met_shape[1]
+            if passthrough.numel() != 3 * n or not passthrough.is_contiguous():
+                raise ValueError("passthrough must be a contiguous tensor of 3*met_nx*met_ny floats")
+            pp = passthrough.data_ptr()
+        self._chk(self._lib.fp_pbl_from_surface(self._h, C.byref(o), int(now_s), 1 if strict else 0, pp))
+
+    def download_pbl(self) -> dict:
+        """Current ustar, wstar, hmix, oli (fp_pbl_download)."""
+        n = self.met_shape[0] * self.met_shape[1]
+        out = {k: np.empty(n, np.float32) for k in ("ustar", "wstar", "hmix", "oli")}
+        self._chk(self._lib.fp_pbl_download(self._h, *[_fp(out[k]) for k in ("ustar", "wstar", "hmix", "oli")]))
+        return out
 
     def upload_forcing(self, dry_vd=None, wet_lambda=None, wet_fraction=None):
         conv = [None if a is None else _f32(a, self.capacity) for a in (dry_vd, wet_lambda, wet_fraction)]

@@ -136,7 +136,7 @@ void destroy(fp_ctx *ctx)
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     fr(ctx->soa[0].base); fr(ctx->soa[1].base);
     fr(ctx->met); fr(ctx->level_heights); fr(ctx->met_stage);
-    fr(ctx->pbl); fr(ctx->precip); fr(ctx->pbl_stage);
+    fr(ctx->pbl); fr(ctx->precip); fr(ctx->pbl_stage); fr(ctx->surf[0]); fr(ctx->surf[1]);
     fr(ctx->dry_vd); fr(ctx->wet_lambda); fr(ctx->wet_fraction);
     fr(ctx->grid_counts); fr(ctx->grid_mass);
     fr(ctx->aos_stage); fr(ctx->slot_stage);
@@ -464,6 +464,71 @@ int fp_pbl_upload(fp_ctx *ctx, const float *ustar, const float *wstar, const flo
     }
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pbl_ready = true;
+    return FP_OK;
+}
+
+static int surface_upload(fp_ctx *ctx, int level, const float *const fields[FP_SURF_COUNT], int64_t time_s,
+                          hipMemcpyKind kind)
+{
+    FP_CHECK(ctx, ctx && fields, "null argument");
+    FP_CHECK(ctx, level == 0 || level == 1, "level must be 0 (t0) or 1 (t1)");
+    for (int k = 0; k < FP_SURF_COUNT; ++k) FP_CHECK(ctx, fields[k], "null surface field");
+    const uint64_t np = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny;
+    if (!ctx->surf[level]) FP_HIP(ctx, dalloc(&ctx->surf[level], np * FP_SURF_COUNT));
+    ctx->surf_ready[level] = false;
+    for (int k = 0; k < FP_SURF_COUNT; ++k)
+        FP_HIP(ctx, hipMemcpyAsync(ctx->surf[level] + k * np, fields[k], np * 4, kind, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->surf_time[level] = time_s;
+    ctx->surf_ready[level] = true;
+    return FP_OK;
+}
+
+int fp_surface_upload(fp_ctx *ctx, int level, const float *const fields[FP_SURF_COUNT], int64_t time_s)
+{
+    return surface_upload(ctx, level, fields, time_s, hipMemcpyHostToDevice);
+}
+
+int fp_surface_upload_dev(fp_ctx *ctx, int level, const float *const fields[FP_SURF_COUNT], int64_t time_s)
+{
+    return surface_upload(ctx, level, fields, time_s, hipMemcpyDeviceToDevice);
+}
+
+int fp_pbl_from_surface(fp_ctx *ctx, const fp_pbl_options *options, int64_t now_s, int strict,
+                        float *passthrough_dev)
+{
+    FP_CHECK(ctx, ctx && options, "null argument");
+    FP_CHECK(ctx, ctx->surf_ready[0] && ctx->surf_ready[1], "surface fields of both bracket ends are required");
+    float alpha = 0.0f;
+    if (fp_interpolation_alpha(ctx->surf_time[0], ctx->surf_time[1], now_s, strict, &alpha) != FP_OK)
+        return fail(ctx, FP_E_INVALID, fp_last_error(nullptr));
+    const uint64_t np = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny;
+    const float *s0[FP_SURF_COUNT], *s1[FP_SURF_COUNT];
+    for (int k = 0; k < FP_SURF_COUNT; ++k) {
+        s0[k] = ctx->surf[0] + k * np;
+        s1[k] = ctx->surf[1] + k * np;
+    }
+    ctx->pbl_ready = false;
+    FP_HIP(ctx, fp::launch_pbl_surface(s0, s1, alpha, *options, ctx->pbl, passthrough_dev, (uint32_t)np, ctx->stream));
+    ctx->pbl_ready = true;  // ordered before the next step on the context's stream
+    return FP_OK;
+}
+
+int fp_pbl_download(fp_ctx *ctx, float *ustar, float *wstar, float *hmix, float *oli)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    FP_CHECK(ctx, ctx->pbl_ready, "no PBL state (fp_pbl_upload or fp_pbl_from_surface first)");
+    const uint64_t np = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny;
+    std::vector<float4> rec(2 * np);
+    FP_HIP(ctx, hipMemcpyAsync(rec.data(), ctx->pbl, 2 * np * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint64_t c = 0; c < np; ++c) {
+        const float4 r = rec[2 * c];
+        if (ustar) ustar[c] = r.x;
+        if (wstar) wstar[c] = r.y;
+        if (hmix) hmix[c] = r.z;
+        if (oli) oli[c] = r.w;
+    }
     return FP_OK;
 }
 

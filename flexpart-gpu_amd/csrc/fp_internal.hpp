@@ -159,6 +159,11 @@ struct fp_ctx {
     bool precip_ready = false;
     float *pbl_stage = nullptr;    // 4 * nx*ny
 
+    // surface fields of the two bracket ends (FP_SURF_COUNT * nx*ny floats each)
+    float *surf[2] = {nullptr, nullptr};
+    int64_t surf_time[2] = {0, 0};
+    bool surf_ready[2] = {false, false};
+
     // forcing
     float *dry_vd = nullptr, *wet_lambda = nullptr, *wet_fraction = nullptr;
 
@@ -234,6 +239,9 @@ hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *re
                                    uint64_t *released, int64_t now, int32_t time, const uint32_t *slots_dev,
                                    uint64_t first_slot, uint64_t max_particles, const Soa &s, void **jobs_dev,
                                    uint64_t *jobs_cap_bytes, hipStream_t st, uint64_t *injected, int *status);
+// PBL records from the two surface bracket ends (fp_surface.hip).
+hipError_t launch_pbl_surface(const float *const s0[FP_SURF_COUNT], const float *const s1[FP_SURF_COUNT], float alpha,
+                              const fp_pbl_options &o, float4 *pbl, float *pass, uint32_t n, hipStream_t st);
 // Adjoint of one step (fp_adj.hip): lam (7 x stride floats: x, y, z, u', v',
 // w', m0 of the step's output) is replaced by the adjoint of its input; au/av
 // (nx*ny doubles, may be null) accumulate the adjoint of the horizontal wind.

@@ -160,6 +160,59 @@ int fp_met_upload_bracket_dev(fp_ctx *ctx, const float *u0, const float *v0, con
  * met_nx x met_ny, index i*ny+j; precip [mm/h] may be NULL. */
 int fp_pbl_upload(fp_ctx *ctx, const float *ustar, const float *wstar, const float *hmix,
                   const float *oli, const float *precip);
+/* ---- PBL from surface fields on the device ---------------------------- */
+
+/* The ten surface fields the PBL diagnostics read, in the order of the
+ * reference's packed SurfaceCellInput (shaders/pbl_diagnostics.wgsl:30-43;
+ * SurfaceFields, wind/mod.rs:194-225): met_nx x met_ny floats each, index
+ * i*ny+j. */
+#define FP_SURF_PRESSURE 0      /* surface pressure [Pa] */
+#define FP_SURF_T2M 1           /* 2 m temperature [K] */
+#define FP_SURF_U10 2           /* 10 m wind [m/s] */
+#define FP_SURF_V10 3
+#define FP_SURF_STRESS 4        /* surface stress [N/m^2] */
+#define FP_SURF_SSHF 5          /* sensible heat flux [W/m^2], positive upward */
+#define FP_SURF_SSR 6           /* solar radiation [W/m^2] */
+#define FP_SURF_HMIX 7          /* met mixing height [m] (<= 0: not provided) */
+#define FP_SURF_USTAR 8         /* met friction velocity [m/s] (<= 0 / non-finite: not provided) */
+#define FP_SURF_OLI 9           /* met 1/L [1/m] (|.| < 1e-5 / non-finite: not provided) */
+#define FP_SURF_COUNT 10
+
+/* PblComputationOptions (io/pbl_params.rs:36-71), the part the GPU
+ * diagnostics read (pbl_diagnostics.wgsl:45-58).  Reference defaults: 0.1,
+ * 10, 1, 800, 100, 4500. */
+typedef struct fp_pbl_options {
+    float roughness_length_m;
+    float wind_reference_height_m;
+    float heat_flux_neutral_threshold_w_m2;
+    float fallback_mixing_height_m;
+    float hmix_min_m;
+    float hmix_max_m;
+} fp_pbl_options;
+
+/* The surface fields of met time level `level` (0: t0, 1: t1) valid at
+ * time_s, kept on the device until replaced (met bracket changes):
+ * SurfaceFieldBuffer::upload (gpu/pbl.rs:208-229) with both bracket ends
+ * resident.  fields[k] is field FP_SURF_k; host pointers (_dev: device
+ * pointers, copied on the context's stream). */
+int fp_surface_upload(fp_ctx *ctx, int level, const float *const fields[FP_SURF_COUNT], int64_t time_s);
+int fp_surface_upload_dev(fp_ctx *ctx, int level, const float *const fields[FP_SURF_COUNT], int64_t time_s);
+/* One kernel per cell: interpolate_surface_fields_linear (io/temporal.rs:
+ * 122-214; alpha as fp_interpolation_alpha with `strict`, exact copies at
+ * alpha <= 0 and >= 1) followed by the PBL diagnostics of
+ * pbl_diagnostics.wgsl:220-305 (timeloop.rs:961-993 with GPU PBL).  The
+ * result replaces the context's PBL state as fp_pbl_upload would (precip is
+ * left unchanged).  passthrough_dev (device, NULL to skip) receives
+ * 3 x met_nx*met_ny floats: sanitized sshf, ssr and surface stress (the
+ * reference's out_sshf / out_ssr / out_surfstr).  Asynchronous on the
+ * context's stream (the next fp_step is ordered after it); fp_sync before
+ * reading passthrough_dev from another stream. */
+int fp_pbl_from_surface(fp_ctx *ctx, const fp_pbl_options *options, int64_t now_s, int strict,
+                        float *passthrough_dev);
+/* PblBuffers::download_state (gpu/pbl.rs): the context's current ustar,
+ * wstar, hmix and 1/L (host arrays of met_nx*met_ny floats; any may be NULL). */
+int fp_pbl_download(fp_ctx *ctx, float *ustar, float *wstar, float *hmix, float *oli);
+
 /* DryDepositionIoBuffers / WetDepositionIoBuffers uploads: per-slot forcing
  * (capacity floats each; NULL leaves that array unchanged). */
 int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,

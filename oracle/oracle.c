@@ -911,3 +911,121 @@ uint64_t orc_release_inject(const orc_grid_domain *grid, const orc_release *rel,
     }
     return w;
 }
+
+/* ---------------------------------------------------------------------- */
+/* PBL from surface fields: io/temporal.rs:122-214 + pbl_diagnostics.wgsl   */
+/* ---------------------------------------------------------------------- */
+
+#define PD_R_AIR 287.05f /* lib.rs:24 */
+#define PD_GA 9.81f      /* lib.rs:26 */
+#define PD_CPA 1004.6f   /* lib.rs:28 */
+#define PD_KARMAN 0.4f
+#define PD_USTAR_MIN 1.0e-4f
+#define PD_NEUTRAL_OLI 1.0e-5f
+
+/* is_finite_scalar and the sanitize_* helpers (pbl_diagnostics.wgsl:98-121) */
+static int pd_finite(float v) { return v == v && fabsf(v) <= FLT_MAX; }
+static float pd_positive(float v, float fb) { return (pd_finite(v) && v > 0.0f) ? v : fb; }
+static float pd_nonneg(float v, float fb) { return (pd_finite(v) && v >= 0.0f) ? v : fb; }
+static float pd_fin(float v, float fb) { return pd_finite(v) ? v : fb; }
+
+/* estimate_friction_velocity (pbl_diagnostics.wgsl:130-152) */
+static float pd_friction_velocity(float stress, float air_density, float u10, float v10, float z0_in,
+                                  float zref_in)
+{
+    const float rho = pd_positive(air_density, 1.225f);
+    const float safe_stress = pd_nonneg(stress, 0.0f);
+    if (safe_stress > 0.0f) return o_max(o_sqrt(safe_stress / rho), PD_USTAR_MIN);
+    const float z0 = pd_positive(z0_in, 0.1f);
+    const float zref = o_max(pd_positive(zref_in, 10.0f), 1.01f * z0);
+    const float speed = o_max(o_sqrt(u10 * u10 + v10 * v10), 0.0f);
+    if (speed <= 0.0f) return PD_USTAR_MIN;
+    const float ustar = PD_KARMAN * speed / o_max(O_LOG(zref / z0), 1.0e-6f);
+    return o_max(pd_positive(ustar, PD_USTAR_MIN), PD_USTAR_MIN);
+}
+
+/* obukhov_length_from_flux (pbl_diagnostics.wgsl:160-187) */
+static float pd_obukhov_length(float ustar, float temp_k, float air_density, float heat_flux, float threshold)
+{
+    const float h = pd_fin(heat_flux, 0.0f);
+    if (fabsf(h) < o_max(threshold, 0.0f)) return INFINITY;
+    const float u = o_max(pd_positive(ustar, PD_USTAR_MIN), PD_USTAR_MIN);
+    const float t = pd_positive(temp_k, 300.0f);
+    const float d = pd_positive(air_density, 1.225f);
+    const float numerator = -(d * PD_CPA * t * u * u * u);
+    const float denominator = PD_KARMAN * PD_GA * h;
+    if (fabsf(denominator) < 1.0e-9f) return INFINITY;
+    const float ol = numerator / denominator;
+    if (!pd_finite(ol) || fabsf(ol) > 1.0e9f) return INFINITY;
+    return ol;
+}
+
+/* inverse_obukhov_from_length (pbl_diagnostics.wgsl:190-195) */
+static float pd_inverse_obukhov(float ol)
+{
+    if (!pd_finite(ol) || fabsf(ol) > 1.0e5f) return 0.0f;
+    return 1.0f / ol;
+}
+
+/* clamp_mixing_height (pbl_diagnostics.wgsl:197-199): max(clamp(v, lo, hi), lo) */
+static float pd_clamp_hmix(float v, const orc_pbl_options *o)
+{
+    return o_max(o_min(o_max(v, o->hmix_min_m), o->hmix_max_m), o->hmix_min_m);
+}
+
+/* convective_velocity_scale (pbl_diagnostics.wgsl:204-218) */
+static float pd_convective_velocity(float heat_flux, float air_density, float temp_k, float hmix,
+                                    const orc_pbl_options *o)
+{
+    if (heat_flux <= 0.0f) return 0.0f;
+    const float d = pd_positive(air_density, 1.225f);
+    const float t = pd_positive(temp_k, 300.0f);
+    const float h = pd_positive(hmix, o->hmix_min_m);
+    const float buoyancy = heat_flux / (d * PD_CPA) * PD_GA / t;
+    if (buoyancy <= 0.0f || !pd_finite(buoyancy)) return 0.0f;
+    return o_max(O_POW(buoyancy * h, 1.0f / 3.0f), 0.0f);
+}
+
+void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], const float *const s1[ORC_SURF_COUNT],
+                          float alpha, const orc_pbl_options *o, float *const out[7])
+{
+    for (uint64_t c = 0; c < n; ++c) {
+        /* interpolate_array2_linear / lerp (io/temporal.rs:290-303); the
+         * endpoints are exact copies (:134-139) */
+        float f[ORC_SURF_COUNT];
+        for (int k = 0; k < ORC_SURF_COUNT; ++k) {
+            const float a = s0[k][c], b = s1[k][c];
+            if (alpha <= 0.0f) f[k] = a;
+            else if (alpha >= 1.0f) f[k] = b;
+            else f[k] = fmaf(a, 1.0f - alpha, b * alpha);
+        }
+        const float ps = f[0], t2m = f[1], u10 = f[2], v10 = f[3], stress = f[4], sshf = f[5], ssr = f[6],
+                    hmix_in = f[7], ustar_in = f[8], oli_in = f[9];
+        /* main (pbl_diagnostics.wgsl:220-305) */
+        const float temp_k = pd_positive(t2m, 300.0f);
+        const float pressure = pd_positive(ps, 101325.0f);
+        const float air_density = pressure / (PD_R_AIR * temp_k);
+        const float estimated = pd_friction_velocity(stress, air_density, u10, v10, o->roughness_length_m,
+                                                     o->wind_reference_height_m);
+        float ustar;
+        if (pd_finite(ustar_in) && ustar_in > 0.0f) ustar = o_max(ustar_in, PD_USTAR_MIN);
+        else ustar = o_max(estimated, PD_USTAR_MIN);
+        const float computed_ol = pd_obukhov_length(ustar, temp_k, air_density, sshf,
+                                                    o->heat_flux_neutral_threshold_w_m2);
+        float oli;
+        if (pd_finite(oli_in) && fabsf(oli_in) >= PD_NEUTRAL_OLI) oli = oli_in;
+        else oli = pd_inverse_obukhov(computed_ol);
+        float hmix;
+        if (pd_finite(hmix_in) && hmix_in > 0.0f) hmix = pd_clamp_hmix(hmix_in, o);
+        else hmix = pd_clamp_hmix(o->fallback_mixing_height_m, o);
+        const float safe_sshf = pd_fin(sshf, 0.0f);
+        const float wstar = pd_convective_velocity(safe_sshf, air_density, temp_k, hmix, o);
+        out[0][c] = ustar;
+        out[1][c] = wstar;
+        out[2][c] = hmix;
+        out[3][c] = oli;
+        out[4][c] = safe_sshf;
+        out[5][c] = pd_nonneg(ssr, 0.0f);
+        out[6][c] = pd_nonneg(stress, 0.0f);
+    }
+}

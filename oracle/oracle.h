@@ -131,6 +131,24 @@ uint64_t orc_release_inject(const orc_grid_domain *grid, const orc_release *rel,
                             uint64_t *released, int64_t now_s, int32_t particle_time,
                             orc_particle *out, uint64_t max_out, int *status);
 
+/* PBL options read by the GPU diagnostics (PblComputationOptions,
+ * io/pbl_params.rs:36-71; pbl_diagnostics.wgsl:45-58). */
+typedef struct {
+    float roughness_length_m, wind_reference_height_m, heat_flux_neutral_threshold_w_m2;
+    float fallback_mixing_height_m, hmix_min_m, hmix_max_m;
+} orc_pbl_options;
+
+/* Surface fields in the order of SurfaceCellInput (pbl_diagnostics.wgsl:30-43):
+ * pressure, T2m, u10, v10, stress, sshf, ssr, hmix, ustar, 1/L. */
+#define ORC_SURF_COUNT 10
+
+/* interpolate_surface_fields_linear (io/temporal.rs:122-214) of the ten
+ * fields with weight alpha, then the GPU PBL diagnostics per cell
+ * (pbl_diagnostics.wgsl:220-305).  out[0..6]: ustar, wstar, hmix, oli, sshf,
+ * ssr, surfstr (n floats each). */
+void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], const float *const s1[ORC_SURF_COUNT],
+                          float alpha, const orc_pbl_options *o, float *const out[7]);
+
 #ifdef __cplusplus
 }
 #endif

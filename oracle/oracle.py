@@ -391,3 +391,39 @@ def release_inject(grid: dict, releases: list, released, now_s: int, particle_ti
            out.ctypes.data, max_out, C.byref(st))
     released[:] = rel
     return out[:n].copy(), int(st.value)
+
+
+class PblOptions(C.Structure):
+    _fields_ = [("roughness_length_m", C.c_float), ("wind_reference_height_m", C.c_float),
+                ("heat_flux_neutral_threshold_w_m2", C.c_float), ("fallback_mixing_height_m", C.c_float),
+                ("hmix_min_m", C.c_float), ("hmix_max_m", C.c_float)]
+
+
+PBL_DEFAULTS = dict(roughness_length_m=0.1, wind_reference_height_m=10.0, heat_flux_neutral_threshold_w_m2=1.0,
+                    fallback_mixing_height_m=800.0, hmix_min_m=100.0, hmix_max_m=4500.0)
+PBL_OUTPUTS = ("ustar", "wstar", "hmix", "oli", "sshf", "ssr", "surfstr")
+
+
+def pbl_from_surface(s0, s1, alpha: float, options: dict | None = None, variant="fpm") -> dict:
+    """interpolate_surface_fields_linear (io/temporal.rs:122-214) + the GPU PBL
+    diagnostics (pbl_diagnostics.wgsl:220-305).  s0, s1: sequences of the ten
+    surface fields (pressure, T2m, u10, v10, stress, sshf, ssr, hmix, ustar,
+    1/L) of equal length.  Returns the seven outputs by name."""
+    a0 = [np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1)) for x in s0]
+    a1 = [np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1)) for x in s1]
+    if len(a0) != 10 or len(a1) != 10:
+        raise ValueError("ten surface fields per level")
+    n = a0[0].size
+    if any(x.size != n for x in a0 + a1):
+        raise ValueError("surface fields differ in size")
+    o = PblOptions(**{**PBL_DEFAULTS, **(options or {})})
+    out = {k: np.empty(n, np.float32) for k in PBL_OUTPUTS}
+    p0 = (C.c_void_p * 10)(*[x.ctypes.data for x in a0])
+    p1 = (C.c_void_p * 10)(*[x.ctypes.data for x in a1])
+    po = (C.c_void_p * 7)(*[out[k].ctypes.data for k in PBL_OUTPUTS])
+    L = lib(variant)
+    fn = L.orc_pbl_from_surface
+    fn.restype = None
+    fn.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_float, C.POINTER(PblOptions), C.c_void_p]
+    fn(n, p0, p1, alpha, C.byref(o), po)
+    return out
