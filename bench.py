@@ -55,6 +55,20 @@ def _traffic_bytes():
     return float(m.group(1)) if m else None
 
 
+def _copy_probe(mib: int):
+    """One device-to-device copy of `mib` MiB: a kernel of known traffic that
+    calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    if not mib:
+        return
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def cpu_baseline(w, threads: int, sample_every: int = 5, steps: int = 10):
     """The oracle on every ``sample_every``-th particle of the workload: one
     warm-up step, then ``steps`` timed steps (median step time), with OpenMP
@@ -111,9 +125,11 @@ def main(argv=None) -> int:
     ap.add_argument("--loss-fraction", type=float, default=0.3,
                     help="stress workload: a particle is lost once its mass falls below this "
                          "fraction of the common initial particle mass")
-    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress"), default="global_10m",
-                    help="global_10m (the headline line), the 96-member batched ensemble, or the "
-                         "differentiable 2M-particle run (adjoint/forward time and FD checks)")
+    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress", "pbl"),
+                    default="global_10m",
+                    help="global_10m (the headline line), the 96-member batched ensemble, the "
+                         "differentiable 2M-particle run (adjoint/forward time and FD checks), the "
+                         "50M deposition stress run, or the per-step PBL-from-surface kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -135,6 +151,9 @@ def main(argv=None) -> int:
     if args.workload == "stress":
         args.steps = 144 if args.steps is None else args.steps
         return bench_stress(args)
+    if args.workload == "pbl":
+        args.steps = 200 if args.steps is None else args.steps
+        return bench_pbl(args)
     args.steps = 144 if args.steps is None else args.steps
 
     import numpy as np
@@ -149,13 +168,7 @@ def main(argv=None) -> int:
     print(f"# workload generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda:0")
-    if args.copy_probe_mib:
-        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    _copy_probe(args.copy_probe_mib)
     levels = [[torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in lvl] for lvl in w.met.levels]
     torch.cuda.synchronize()
 
@@ -689,6 +702,82 @@ def bench_stress(args) -> int:
         },
         "roofline": None,
         "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_pbl(args) -> int:
+    """The step before the particle step: surface temporal interpolation and
+    PBL diagnostics for the whole met grid (fp_pbl_from_surface), once per
+    600 s step, on the 0.25 deg grid of the stress run (1440 x 721) and the
+    1 deg grid of the 10M run (360 x 181).  Kernel time from HIP events on
+    the library stream; algorithmic bytes per cell = 2 x 10 surface floats
+    read + two 16-byte PBL records written = 112 B.  CPU baseline: the C
+    oracle (one thread) on the same grid, the reference's own per-step CPU
+    work (interpolate_surface_fields_linear + compute_pbl_parameters_from_met)."""
+    import numpy as np
+    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    _copy_probe(args.copy_probe_mib)
+    grids = {}
+    for nx, ny in ((1440, 721), (360, 181)):
+        s0, s1 = W.surface_bracket(nx, ny)
+        ctx = fp.GpuTimestep(64, (nx, ny, 2), n_species=1, device=0)
+        ctx.upload_surface(0, s0, 0)
+        ctx.upload_surface(1, s1, 21_600)
+        opts = fp.PblOptions()
+        for k in range(args.warmup):
+            ctx.pbl_from_surface(opts, 600 * k)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ctx.pbl_from_surface(opts, 600 * (k % 37))
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ms, cnt = ctx.profile_read(fp.KERNEL_PBL)
+        ctx.profile(False)
+        ncell = nx * ny
+        kern_us = ms / cnt * 1e3
+        grids[f"{nx}x{ny}"] = {"cells": ncell, "kernel_us": kern_us, "wall_us_per_call": wall / args.steps * 1e6,
+                               "achieved_GBps": ncell * 112 / (kern_us * 1e-6) / 1e9}
+        ctx.close()
+        if not args.no_cpu_baseline and (nx, ny) == (1440, 721):
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as orc
+            orc.pbl_from_surface(s0, s1, 0.25)
+            times = []
+            while sum(times) < 3.0:
+                t0 = time.perf_counter()
+                orc.pbl_from_surface(s0, s1, 0.25)
+                times.append(time.perf_counter() - t0)
+            cpu_s, reps = float(np.median(times)), len(times)
+    big = grids["1440x721"]
+    result = {
+        "metric": "PBL cells/s (surface interpolation + PBL diagnostics, 1440x721 grid, 1 GPU)",
+        "value": big["cells"] / (big["kernel_us"] * 1e-6),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": big["kernel_us"] / 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pbl_from_surface: ten surface fields x two met levels resident, "
+                               "interpolated and diagnosed per 600 s step", "grids": grids},
+        "roofline": {"bound": "hbm", "achieved": big["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": big["achieved_GBps"] / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None if args.no_cpu_baseline else {
+            "value": big["cells"] / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"1440x721 grid, C oracle, one thread, median of {reps} calls (3 s)"},
     }
     print(json.dumps(result), flush=True)
     return 0

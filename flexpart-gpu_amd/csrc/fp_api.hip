@@ -509,7 +509,10 @@ int fp_pbl_from_surface(fp_ctx *ctx, const fp_pbl_options *options, int64_t now_
         s1[k] = ctx->surf[1] + k * np;
     }
     ctx->pbl_ready = false;
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_PBL);
     FP_HIP(ctx, fp::launch_pbl_surface(s0, s1, alpha, *options, ctx->pbl, passthrough_dev, (uint32_t)np, ctx->stream));
+    prof_end(ctx, ev);
     ctx->pbl_ready = true;  // ordered before the next step on the context's stream
     return FP_OK;
 }

@@ -207,8 +207,8 @@ struct fp_ctx {
     struct Ev { hipEvent_t a, b; int kernel; };
     std::vector<Ev> events;
     std::vector<Ev> free_events;
-    double prof_ms[FP_KERNEL_COUNT] = {0, 0, 0, 0};
-    uint64_t prof_n[FP_KERNEL_COUNT] = {0, 0, 0, 0};
+    double prof_ms[FP_KERNEL_COUNT] = {};
+    uint64_t prof_n[FP_KERNEL_COUNT] = {};
 
     const fp::Soa &s() const { return soa[cur]; }
 };

@@ -397,3 +397,31 @@ def stress_case(n_particles: int = 50_000_000, n_sources: int = 4096, seed: int 
     return AdjointCase(met, pbl, counts.astype(np.uint32), q.astype(np.float32), pos,
                        np.zeros((0, nx * ny), np.float32), np.zeros((0, nx * ny), np.float32),
                        np.zeros(0, np.float32), 600.0, scale, scale, (seed, 0x57E55))
+
+
+def surface_bracket(nx: int, ny: int, seed: int = 11):
+    """Two time levels of the ten surface fields the PBL diagnostics read
+    (FP_SURF_* order: pressure, T2m, u10, v10, stress, sshf, ssr, hmix,
+    met u*, met 1/L), nx*ny each, index i*ny+j: smooth large-scale patterns
+    plus noise, a diurnal heat-flux swing between the two levels, stress
+    missing over a third of the cells (log-law u*), hmix missing over a fifth
+    (fallback) and met u* / 1/L provided over half."""
+    rng = np.random.default_rng(seed)
+    lat = np.linspace(-90.0, 90.0, ny, dtype=np.float64)[None, :] * np.ones((nx, 1))
+    lon = np.linspace(0.0, 360.0, nx, endpoint=False, dtype=np.float64)[:, None] * np.ones((1, ny))
+    levels = []
+    for lvl in range(2):
+        phase = np.deg2rad(lon) + lvl * np.pi / 2
+        t2m = 288.0 - 30.0 * np.sin(np.deg2rad(lat)) ** 2 + rng.normal(0, 2.0, lat.shape)
+        ps = 101_325.0 - 1500.0 * np.cos(3 * np.deg2rad(lat)) + rng.normal(0, 300.0, lat.shape)
+        u10 = 8.0 * np.sin(2 * np.deg2rad(lat)) + rng.normal(0, 3.0, lat.shape)
+        v10 = rng.normal(0, 3.0, lat.shape)
+        stress = np.where(rng.uniform(size=lat.shape) < 1 / 3, 0.0, 0.0012 * 1.2 * (u10 ** 2 + v10 ** 2))
+        sshf = 150.0 * np.sin(phase) * np.cos(np.deg2rad(lat)) + rng.normal(0, 20.0, lat.shape)
+        ssr = np.maximum(0.0, 800.0 * np.sin(phase)) * np.cos(np.deg2rad(lat))
+        hmix = np.where(rng.uniform(size=lat.shape) < 0.2, 0.0, 800.0 * np.exp(rng.normal(0, 0.6, lat.shape)))
+        ust = np.where(rng.uniform(size=lat.shape) < 0.5, 0.0, rng.uniform(0.1, 0.8, lat.shape))
+        oli = np.where(rng.uniform(size=lat.shape) < 0.5, 0.0, rng.uniform(-0.02, 0.02, lat.shape))
+        levels.append([np.ascontiguousarray(x.reshape(-1).astype(np.float32))
+                       for x in (ps, t2m, u10, v10, stress, sshf, ssr, hmix, ust, oli)])
+    return levels
