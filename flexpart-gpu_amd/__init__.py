@@ -155,6 +155,17 @@ class PblOptions(C.Structure):
         super().__init__(**d)
 
 
+class OutputGeometry(C.Structure):
+    """fp_output_geometry: south-west corner and cell size of the output grid [deg]."""
+    _fields_ = [("xlon0", C.c_double), ("ylat0", C.c_double), ("dx", C.c_double), ("dy", C.c_double)]
+
+
+class NetcdfSnapshot(C.Structure):
+    _fields_ = [("time_s", C.c_int64), ("species_index", C.c_int32), ("nx", C.c_uint32), ("ny", C.c_uint32),
+                ("nz", C.c_uint32), ("count", C.c_void_p), ("mass_kg", C.c_void_p), ("dry_kg_m2", C.c_void_p),
+                ("wet_kg_m2", C.c_void_p), ("conc_kg_m3", C.c_void_p)]
+
+
 # the ten surface fields of fp_surface_upload, in FP_SURF_* order
 SURFACE_FIELDS = ("surface_pressure_pa", "temperature_2m_k", "u10_ms", "v10_ms", "surface_stress_n_m2",
                   "sensible_heat_flux_w_m2", "solar_radiation_w_m2", "mixing_height_m", "friction_velocity_ms",
@@ -188,6 +199,10 @@ ABI = {
     "fp_surface_upload_dev": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int64]),
     "fp_pbl_from_surface": (C.c_int, [_vp, C.POINTER(PblOptions), C.c_int64, C.c_int, _vp]),
     "fp_pbl_download": (C.c_int, [_vp, _f, _f, _f, _f]),
+    "fp_output_reset": (C.c_int, [_vp]),
+    "fp_output_sample": (C.c_int, [_vp]),
+    "fp_output_finalize": (C.c_int, [_vp, C.POINTER(OutputGeometry), _u32, _f, _f, _u32]),
+    "fp_write_netcdf": (C.c_int, [C.c_char_p, C.POINTER(NetcdfSnapshot), C.c_int]),
     "fp_step": (C.c_int, [_vp, C.POINTER(StepParams), _u32]),
     "fp_step_vjp": (C.c_int, [_vp, C.POINTER(StepParams), _f, _f, _d, _d]),
     "fp_grid_clear": (C.c_int, [_vp]),
@@ -452,6 +467,29 @@ class GpuTimestep:
         self._chk(self._lib.fp_pbl_download(self._h, *[_fp(out[k]) for k in ("ustar", "wstar", "hmix", "oli")]))
         return out
 
+    # -- concentration output intervals
+    def output_reset(self):
+        self._chk(self._lib.fp_output_reset(self._h))
+
+    def output_sample(self):
+        """Add the current concentration grid to the interval sums (fp_output_sample)."""
+        self._chk(self._lib.fp_output_sample(self._h))
+
+    def output_finalize(self, geometry: "OutputGeometry | None" = None, concentration: bool = False) -> dict:
+        """Interval fields (fp_output_finalize): count (u32), mass_kg (mean),
+        and conc_kg_m3 when concentration=True (needs geometry)."""
+        n = int(np.prod(self.out_shape))
+        count = np.empty(n, np.uint32)
+        mass = np.empty(n, np.float32)
+        conc = np.empty(n, np.float32) if concentration else None
+        ns = C.c_uint32()
+        self._chk(self._lib.fp_output_finalize(self._h, None if geometry is None else C.byref(geometry),
+                                               count.ctypes.data_as(_u32), _fp(mass), _fp(conc), C.byref(ns)))
+        out = {"count": count, "mass_kg": mass, "n_samples": int(ns.value)}
+        if conc is not None:
+            out["conc_kg_m3"] = conc
+        return out
+
     def upload_forcing(self, dry_vd=None, wet_lambda=None, wet_fraction=None):
         conv = [None if a is None else _f32(a, self.capacity) for a in (dry_vd, wet_lambda, wet_fraction)]
         self._chk(self._lib.fp_forcing_upload(self._h, *[_fp(a) for a in conv]))
@@ -596,6 +634,25 @@ class GpuTimestep:
         return {"block_size": r.block_size, "particles_per_thread": r.particles_per_thread,
                 "sort_interval": r.sort_interval, "from_cache": bool(r.from_cache),
                 "step_ms": r.step_ms, "step_ms_with_sort": r.step_ms_with_sort}
+
+
+def write_netcdf(path: str, shape, count, mass_kg, time_s: int, species_index: int = 0, dry_kg_m2=None,
+                 wet_kg_m2=None, conc_kg_m3=None, fmt: int = 5):
+    """write_gridded_output_netcdf (io/netcdf_output.rs:141-229) through
+    fp_write_netcdf: fmt 5 = CDF-5 (the reference's types), 2 = CDF-2."""
+    lib = library()
+    nx, ny, nz = (int(v) for v in shape)
+    cnt = np.ascontiguousarray(np.asarray(count, np.uint32).reshape(-1))
+    arrs = [_f32(mass_kg, nx * ny * nz)]
+    if cnt.size != nx * ny * nz:
+        raise ValueError("count must have nx*ny*nz values")
+    opt = [None if a is None else _f32(a, n) for a, n in ((dry_kg_m2, nx * ny), (wet_kg_m2, nx * ny),
+                                                          (conc_kg_m3, nx * ny * nz))]
+    s = NetcdfSnapshot(int(time_s), int(species_index), nx, ny, nz, cnt.ctypes.data, arrs[0].ctypes.data,
+                       *[None if a is None else a.ctypes.data for a in opt])
+    rc = lib.fp_write_netcdf(os.fsencode(path), C.byref(s), int(fmt))
+    if rc != FP_OK:
+        raise FpError(rc, lib.fp_last_error(None).decode())
 
 
 def interpolation_alpha(t0_s: int, t1_s: int, target_s: int, strict: bool = False) -> float:

@@ -137,6 +137,8 @@ void destroy(fp_ctx *ctx)
     fr(ctx->soa[0].base); fr(ctx->soa[1].base);
     fr(ctx->met); fr(ctx->level_heights); fr(ctx->met_stage);
     fr(ctx->pbl); fr(ctx->precip); fr(ctx->pbl_stage); fr(ctx->surf[0]); fr(ctx->surf[1]);
+    fr(ctx->out_acc_count); fr(ctx->out_acc_mass); fr(ctx->out_count); fr(ctx->out_mass); fr(ctx->out_conc);
+    fr(ctx->out_vol);
     fr(ctx->dry_vd); fr(ctx->wet_lambda); fr(ctx->wet_fraction);
     fr(ctx->grid_counts); fr(ctx->grid_mass);
     fr(ctx->aos_stage); fr(ctx->slot_stage);
@@ -532,6 +534,100 @@ int fp_pbl_download(fp_ctx *ctx, float *ustar, float *wstar, float *hmix, float 
         if (hmix) hmix[c] = r.z;
         if (oli) oli[c] = r.w;
     }
+    return FP_OK;
+}
+
+static int output_alloc(fp_ctx *ctx)
+{
+    const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    if (!ctx->out_acc_count) {
+        FP_HIP(ctx, dalloc(&ctx->out_acc_count, ncell));
+        FP_HIP(ctx, dalloc(&ctx->out_acc_mass, ncell));
+        FP_HIP(ctx, hipMemsetAsync(ctx->out_acc_count, 0, ncell * 8, ctx->stream));
+        FP_HIP(ctx, hipMemsetAsync(ctx->out_acc_mass, 0, ncell * 8, ctx->stream));
+        ctx->out_samples = 0;
+    }
+    return FP_OK;
+}
+
+int fp_output_reset(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    if (int rc = output_alloc(ctx)) return rc;
+    const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    FP_HIP(ctx, hipMemsetAsync(ctx->out_acc_count, 0, ncell * 8, ctx->stream));
+    FP_HIP(ctx, hipMemsetAsync(ctx->out_acc_mass, 0, ncell * 8, ctx->stream));
+    ctx->out_samples = 0;
+    return FP_OK;
+}
+
+int fp_output_sample(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    FP_CHECK(ctx, ctx->out_samples < 0xffffffffu, "too many samples in one output interval");
+    if (int rc = output_alloc(ctx)) return rc;
+    const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    FP_HIP(ctx, fp::launch_output_sample(ctx->grid_counts, ctx->grid_mass, ctx->out_acc_count, ctx->out_acc_mass,
+                                         ncell, ctx->stream));
+    ++ctx->out_samples;
+    return FP_OK;
+}
+
+int fp_output_finalize(fp_ctx *ctx, const fp_output_geometry *geometry, uint32_t *count_out, float *mass_out,
+                       float *conc_out, uint32_t *n_samples)
+{
+    FP_CHECK(ctx, ctx && ctx->grid_counts, "context has no output grid");
+    FP_CHECK(ctx, ctx->out_samples > 0, "no samples in the output interval");
+    FP_CHECK(ctx, count_out && mass_out, "null argument");
+    const uint32_t nx = ctx->desc.out_nx, ny = ctx->desc.out_ny, nz = ctx->desc.out_nz;
+    const uint64_t ncell = (uint64_t)nx * ny * nz;
+    const float *oh = ctx->grid_params.outheights;
+    if (conc_out) {
+        FP_CHECK(ctx, geometry, "concentrations need the output grid geometry");
+        FP_CHECK(ctx, std::isfinite(geometry->dx) && std::isfinite(geometry->dy) && geometry->dx > 0.0 &&
+                          geometry->dy > 0.0 && std::isfinite(geometry->ylat0),
+                 "output cell size must be positive and finite");
+        FP_CHECK(ctx, oh[nz - 1] > 0.0f, "concentrations need metre output heights");
+        for (uint32_t k = 0; k < nz; ++k)
+            FP_CHECK(ctx, oh[k] > (k ? oh[k - 1] : 0.0f), "output heights must increase from 0");
+    }
+    if (!ctx->out_count) {
+        FP_HIP(ctx, dalloc(&ctx->out_count, ncell));
+        FP_HIP(ctx, dalloc(&ctx->out_mass, ncell));
+    }
+    if (conc_out) {
+        if (!ctx->out_conc) {
+            FP_HIP(ctx, dalloc(&ctx->out_conc, ncell));
+            FP_HIP(ctx, dalloc(&ctx->out_vol, (uint64_t)ny * nz));
+        }
+        std::vector<float> vol((uint64_t)ny * nz);
+        fp::output_volumes(*geometry, oh, ny, nz, vol.data());
+        FP_HIP(ctx, hipMemcpyAsync(ctx->out_vol, vol.data(), vol.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    FP_HIP(ctx, fp::launch_output_finalize(ctx->out_acc_count, ctx->out_acc_mass, ctx->out_samples,
+                                           ctx->grid_params.mass_scale, ctx->out_vol, ny, nz, ctx->out_count,
+                                           ctx->out_mass, conc_out ? ctx->out_conc : nullptr, ncell, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(count_out, ctx->out_count, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(mass_out, ctx->out_mass, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (conc_out)
+        FP_HIP(ctx, hipMemcpyAsync(conc_out, ctx->out_conc, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n_samples) *n_samples = ctx->out_samples;
+    return FP_OK;
+}
+
+int fp_write_netcdf(const char *path, const fp_netcdf_snapshot *s, int format)
+{
+    if (!path || !s) return fail(nullptr, FP_E_INVALID, "null argument");
+    if (format != 2 && format != 5) return fail(nullptr, FP_E_INVALID, "format must be 2 (CDF-2) or 5 (CDF-5)");
+    if (s->nx == 0 || s->ny == 0 || s->nz == 0) return fail(nullptr, FP_E_INVALID, "grid shape must be non-zero");
+    if (!s->count || !s->mass_kg) return fail(nullptr, FP_E_INVALID, "count and mass_kg are required");
+    if ((uint64_t)s->nx * s->ny * s->nz * 4 > 0xfffffffcull && format == 2)
+        return fail(nullptr, FP_E_INVALID, "variable too large for CDF-2");
+    std::string err;
+    const int rc = fp::write_netcdf(path, *s, format, err);
+    if (rc != FP_OK) return fail(nullptr, rc, err);
     return FP_OK;
 }
 

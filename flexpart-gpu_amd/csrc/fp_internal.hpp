@@ -194,6 +194,12 @@ struct fp_ctx {
     void *sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
 
+    // concentration output: interval sums and finalize buffers
+    unsigned long long *out_acc_count = nullptr, *out_acc_mass = nullptr;
+    uint32_t out_samples = 0;
+    uint32_t *out_count = nullptr;
+    float *out_mass = nullptr, *out_conc = nullptr, *out_vol = nullptr;
+
     // source-term injection plans
     void *release_jobs = nullptr;
     uint64_t release_jobs_bytes = 0;
@@ -242,6 +248,14 @@ hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *re
 // PBL records from the two surface bracket ends (fp_surface.hip).
 hipError_t launch_pbl_surface(const float *const s0[FP_SURF_COUNT], const float *const s1[FP_SURF_COUNT], float alpha,
                               const fp_pbl_options &o, float4 *pbl, float *pass, uint32_t n, hipStream_t st);
+// Concentration output (fp_output.hip).
+hipError_t launch_output_sample(const uint32_t *counts, const uint32_t *mass, unsigned long long *acc_count,
+                                unsigned long long *acc_mass, uint64_t n, hipStream_t st);
+hipError_t launch_output_finalize(const unsigned long long *acc_count, const unsigned long long *acc_mass,
+                                  uint32_t n_samples, float mass_scale, const float *vol, uint32_t ny, uint32_t nz,
+                                  uint32_t *count_out, float *mass_out, float *conc_out, uint64_t n, hipStream_t st);
+void output_volumes(const fp_output_geometry &g, const float *outheights, uint32_t ny, uint32_t nz, float *vol);
+int write_netcdf(const char *path, const fp_netcdf_snapshot &s, int version, std::string &err);
 // Adjoint of one step (fp_adj.hip): lam (7 x stride floats: x, y, z, u', v',
 // w', m0 of the step's output) is replaced by the adjoint of its input; au/av
 // (nx*ny doubles, may be null) accumulate the adjoint of the horizontal wind.

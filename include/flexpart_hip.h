@@ -306,6 +306,53 @@ int fp_release_inject(fp_ctx *ctx, const fp_grid_domain *grid, const fp_release 
                       const uint32_t *slots, uint64_t n_slots, uint64_t first_slot,
                       uint64_t *injected);
 
+/* ---- concentration output ------------------------------------------- */
+
+/* Output grid geometry for the cell volumes (OUTGRID: OUTLON0, OUTLAT0,
+ * DXOUT, DYOUT in degrees; docs/science/concentration-gridding.md:9-20). */
+typedef struct fp_output_geometry {
+    double xlon0, ylat0;   /* south-west corner of the output grid [deg] */
+    double dx, dy;         /* output cell size [deg] */
+} fp_output_geometry;
+
+/* Output intervals: fp_output_sample adds the current concentration grid
+ * (fp_grid_accumulate or a grid_fused step) to per-cell 64-bit sums on the
+ * device; fp_output_finalize turns the sums into the interval fields.  The
+ * reference downloads and converts one instantaneous grid per output time
+ * (accumulate_concentration_grid_gpu, gpu/gridding.rs:194-219); one sample
+ * per interval reproduces that exactly. */
+int fp_output_reset(fp_ctx *ctx);
+int fp_output_sample(fp_ctx *ctx);
+/* count_out: particle samples per cell over the interval (u32, saturating);
+ * mass_out: interval-mean mass [kg] = (f32)(sum / n_samples) / mass_scale;
+ * conc_out (NULL to skip; needs geometry and metre output heights): mass_out
+ * / V_cell [kg m-3], V_cell = (dx * dx_meters_at_latitude(lat_c)) *
+ * (dy * R PI180) * dz (docs/science/concentration-gridding.md:35-41,
+ * coords/mod.rs:156-158).  Host arrays of out_nx*out_ny*out_nz, index
+ * (ix*ny+iy)*nz+iz; n_samples may be NULL.  The sums are kept. */
+int fp_output_finalize(fp_ctx *ctx, const fp_output_geometry *geometry, uint32_t *count_out, float *mass_out,
+                       float *conc_out, uint32_t *n_samples);
+
+/* GriddedOutputSnapshot (io/netcdf_output.rs:43-57) plus the optional
+ * volume-normalised concentration. */
+typedef struct fp_netcdf_snapshot {
+    int64_t time_s;                /* simulation time [s since the epoch] */
+    int32_t species_index;
+    uint32_t nx, ny, nz;
+    const uint32_t *count;         /* nx*ny*nz particle counts */
+    const float *mass_kg;          /* nx*ny*nz */
+    const float *dry_kg_m2;        /* nx*ny (NULL: zeros) */
+    const float *wet_kg_m2;        /* nx*ny (NULL: zeros) */
+    const float *conc_kg_m3;       /* nx*ny*nz or NULL (extra variable concentration_kg_m3) */
+} fp_netcdf_snapshot;
+
+/* write_gridded_output_netcdf (io/netcdf_output.rs:141-229): the same
+ * dimensions (time = 1, x, y, z), variables, types and attributes.
+ * format 5: netCDF-3 64-bit data (CDF-5; u32 counts, i64 time); format 2:
+ * classic 64-bit offset (CDF-2; counts as i32, time as f64) for readers
+ * without CDF-5.  Host only; errors via fp_last_error(NULL). */
+int fp_write_netcdf(const char *path, const fp_netcdf_snapshot *snapshot, int format);
+
 /* ---- temporal weight ------------------------------------------------ */
 
 /* interpolation_alpha (simulation/timeloop.rs:2086-2120): the weight of the

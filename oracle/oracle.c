@@ -1029,3 +1029,44 @@ void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], con
         out[6][c] = pd_nonneg(stress, 0.0f);
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Concentration output: gpu/gridding.rs:194-219, concentration-gridding.md */
+/* ---------------------------------------------------------------------- */
+
+/* dx_meters_at_latitude (coords/mod.rs:156-158), f32 with R_EARTH = 6.371e6
+ * (lib.rs:22) and PI180 = pi/180 (lib.rs:32) */
+static float out_dx_meters(float lat_deg)
+{
+    const float r_earth = 6.371e6f, pi180 = 3.14159265358979323846f / 180.0f;
+    return r_earth * cosf(lat_deg * pi180) * pi180;
+}
+
+void orc_output_volumes(double ylat0, double dx, double dy, const float *outheights, uint32_t ny, uint32_t nz,
+                        float *vol)
+{
+    const double dy_m = (double)out_dx_meters(0.0f); /* R PI180: cos 0 = 1 */
+    for (uint32_t iy = 0; iy < ny; ++iy) {
+        const float lat = (float)(ylat0 + ((double)iy + 0.5) * dy);
+        const double area = (dx * (double)out_dx_meters(lat)) * (dy * dy_m);
+        for (uint32_t k = 0; k < nz; ++k) {
+            const double dz = (double)outheights[k] - (k ? (double)outheights[k - 1] : 0.0);
+            vol[(uint64_t)iy * nz + k] = (float)(area * dz);
+        }
+    }
+}
+
+void orc_output_finalize(uint64_t ncell, const uint64_t *acc_count, const uint64_t *acc_mass, uint32_t n_samples,
+                         float mass_scale, const float *vol, uint32_t ny, uint32_t nz, uint32_t *count, float *mass,
+                         float *conc)
+{
+    for (uint64_t c = 0; c < ncell; ++c) {
+        count[c] = acc_count[c] > 0xffffffffull ? 0xffffffffu : (uint32_t)acc_count[c];
+        const float m = (float)((double)acc_mass[c] / (double)n_samples) / mass_scale;
+        mass[c] = m;
+        if (vol && conc) {
+            const uint32_t k = (uint32_t)(c % nz), iy = (uint32_t)((c / nz) % ny);
+            conc[c] = m / vol[(uint64_t)iy * nz + k];
+        }
+    }
+}

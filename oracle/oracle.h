@@ -149,6 +149,18 @@ typedef struct {
 void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], const float *const s1[ORC_SURF_COUNT],
                           float alpha, const orc_pbl_options *o, float *const out[7]);
 
+/* Concentration output.  Volumes of the output cells per row iy and level k
+ * (docs/science/concentration-gridding.md:35-41 with dx_meters_at_latitude,
+ * coords/mod.rs:156-158): vol[iy*nz+k]. */
+void orc_output_volumes(double ylat0, double dx, double dy, const float *outheights, uint32_t ny, uint32_t nz,
+                        float *vol);
+/* Interval fields from 64-bit sums: count (saturating), mean mass [kg] =
+ * (f32)(sum/n)/mass_scale (gpu/gridding.rs:210-213 for n = 1) and, when vol
+ * is not NULL, mass / vol. */
+void orc_output_finalize(uint64_t ncell, const uint64_t *acc_count, const uint64_t *acc_mass, uint32_t n_samples,
+                         float mass_scale, const float *vol, uint32_t ny, uint32_t nz, uint32_t *count, float *mass,
+                         float *conc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -427,3 +427,36 @@ def pbl_from_surface(s0, s1, alpha: float, options: dict | None = None, variant=
     fn.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_float, C.POINTER(PblOptions), C.c_void_p]
     fn(n, p0, p1, alpha, C.byref(o), po)
     return out
+
+
+def output_volumes(ylat0: float, dx: float, dy: float, outheights, ny: int) -> np.ndarray:
+    """Cell volumes [m^3] per output row and level (orc_output_volumes): (ny, nz)."""
+    oh = np.ascontiguousarray(np.asarray(outheights, np.float32).reshape(-1))
+    vol = np.empty(ny * oh.size, np.float32)
+    fn = lib().orc_output_volumes
+    fn.restype = None
+    fn.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    fn(ylat0, dx, dy, oh.ctypes.data, ny, oh.size, vol.ctypes.data)
+    return vol.reshape(ny, oh.size)
+
+
+def output_finalize(acc_count, acc_mass, n_samples: int, mass_scale: float, shape, vol=None) -> dict:
+    """Interval fields from the 64-bit sums (orc_output_finalize)."""
+    nx, ny, nz = shape
+    ac = np.ascontiguousarray(np.asarray(acc_count, np.uint64).reshape(-1))
+    am = np.ascontiguousarray(np.asarray(acc_mass, np.uint64).reshape(-1))
+    n = ac.size
+    count = np.empty(n, np.uint32)
+    mass = np.empty(n, np.float32)
+    conc = np.empty(n, np.float32) if vol is not None else None
+    v = None if vol is None else np.ascontiguousarray(np.asarray(vol, np.float32).reshape(-1))
+    fn = lib().orc_output_finalize
+    fn.restype = None
+    fn.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p, C.c_uint32, C.c_uint32,
+                   C.c_void_p, C.c_void_p, C.c_void_p]
+    fn(n, ac.ctypes.data, am.ctypes.data, n_samples, mass_scale, None if v is None else v.ctypes.data, ny, nz,
+       count.ctypes.data, mass.ctypes.data, None if conc is None else conc.ctypes.data)
+    out = {"count": count, "mass_kg": mass}
+    if conc is not None:
+        out["conc_kg_m3"] = conc
+    return out
