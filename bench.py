@@ -125,11 +125,12 @@ def main(argv=None) -> int:
     ap.add_argument("--loss-fraction", type=float, default=0.3,
                     help="stress workload: a particle is lost once its mass falls below this "
                          "fraction of the common initial particle mass")
-    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress", "pbl"),
+    ap.add_argument("--workload", choices=("global_10m", "ensemble", "adjoint", "stress", "pbl", "output"),
                     default="global_10m",
                     help="global_10m (the headline line), the 96-member batched ensemble, the "
                          "differentiable 2M-particle run (adjoint/forward time and FD checks), the "
-                         "50M deposition stress run, or the per-step PBL-from-surface kernel")
+                         "50M deposition stress run, the per-step PBL-from-surface kernel, or the "
+                         "concentration output (interval sums, finalize, NetCDF)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -154,6 +155,9 @@ def main(argv=None) -> int:
     if args.workload == "pbl":
         args.steps = 200 if args.steps is None else args.steps
         return bench_pbl(args)
+    if args.workload == "output":
+        args.steps = 144 if args.steps is None else args.steps
+        return bench_output(args)
     args.steps = 144 if args.steps is None else args.steps
 
     import numpy as np
@@ -778,6 +782,107 @@ def bench_pbl(args) -> int:
         "cpu_baseline": None if args.no_cpu_baseline else {
             "value": big["cells"] / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
             "sample": f"1440x721 grid, C oracle, one thread, median of {reps} calls (3 s)"},
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_output(args) -> int:
+    """Concentration output of the 10M run: the 720 x 361 x 5 fused grid of
+    every step added to the 64-bit interval sums (fp_output_sample, 144
+    samples = 24 h), then one finalize to counts, mean mass and cell-volume
+    concentrations, and the CDF-5 snapshot file.  Sample-kernel time from HIP
+    events on the library stream; algorithmic bytes per cell: 8 B of grid
+    read + 16 B of sums read and written = 40 B.  CPU baseline: the same
+    interval sums in numpy on the host (one thread)."""
+    import tempfile
+
+    import numpy as np
+    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+
+    fp = _package()
+    import flexpart_gpu_amd.workloads as W
+
+    _copy_probe(args.copy_probe_mib)
+    w = W.global_10m(1_000_000)
+    nx, ny, nz = w.out_shape
+    ncell = nx * ny * nz
+    ctx = fp.GpuTimestep(len(w.particles), w.met.shape, n_species=1, out_shape=w.out_shape, device=0)
+    u0, v0, w0 = w.met.levels[0]
+    u1, v1, w1 = w.met.levels[1]
+    ctx.upload_met_bracket(u0, v0, w0, u1, v1, w1, w.met.level_heights)
+    ctx.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli)
+    ctx.upload_store(w.particles)
+    ctx.configure_grid(w.outheights, w.refine, 0, w.mass_scale)
+    cfg = fp.StepConfig(dt_seconds=w.dt_seconds, x_scale=w.x_scale, y_scale=w.y_scale, key=w.key, dry_mode=1,
+                        dry_vd=2e-3, grid_fused=True)
+    counter = (0, 0, 0, 0)
+    for k in range(6):  # spread the plumes a little
+        cfg.counter, cfg.alpha = counter, k / 36.0
+        ctx.clear_grid()
+        counter = ctx.step(cfg)
+    ctx.output_reset()
+    for _ in range(args.warmup):
+        ctx.output_sample()
+    ctx.output_reset()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.output_sample()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms, cnt = ctx.profile_read(fp.KERNEL_OUTPUT)
+    ctx.profile(False)
+    sample_us = ms / cnt * 1e3
+    geo = fp.OutputGeometry(xlon0=-180.0, ylat0=-90.0, dx=0.5, dy=0.5)
+    t0 = time.perf_counter()
+    fin = ctx.output_finalize(geo, concentration=True)
+    finalize_ms = (time.perf_counter() - t0) * 1e3
+    counts, mass = ctx.download_grid()
+    ctx.close()
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        fp.write_netcdf(os.path.join(d, "out.nc"), w.out_shape, fin["count"], fin["mass_kg"], 1_706_000_000,
+                        conc_kg_m3=fin["conc_kg_m3"])
+        write_ms = (time.perf_counter() - t0) * 1e3
+        file_mb = os.path.getsize(os.path.join(d, "out.nc")) / 1e6
+    cpu = None
+    if not args.no_cpu_baseline:
+        acc_c = np.zeros(ncell, np.uint64)
+        acc_m = np.zeros(ncell, np.uint64)
+        c32, m32 = counts.reshape(-1), mass.reshape(-1)
+        times = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            acc_c += c32
+            acc_m += m32
+            times.append(time.perf_counter() - t0)
+        cpu = {"value": ncell / float(np.median(times)), "unit": "cells/s", "cores": 1, "kind": "port",
+               "sample": f"720x361x5 grid, numpy u64 interval sums on the host, median of 20 samples"}
+    achieved = ncell * 40 / (sample_us * 1e-6) / 1e9
+    result = {
+        "metric": "output cells/s (interval sums of the 720x361x5 concentration grid, 1 GPU)",
+        "value": ncell / (sample_us * 1e-6),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sample_us / 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "output: 144 step samples of the fused 720x361x5 grid of the 1M-particle global "
+                               "run, finalize with cell-volume concentrations, CDF-5 snapshot",
+                   "cells": ncell, "sample_kernel_us": sample_us, "sample_wall_us": wall / args.steps * 1e6,
+                   "finalize_ms_incl_download": finalize_ms, "netcdf_write_ms": write_ms, "netcdf_mb": file_mb,
+                   "n_samples": fin["n_samples"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
     return 0

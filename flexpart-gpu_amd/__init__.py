@@ -35,7 +35,7 @@ MAX_OUTPUT_LEVELS = 16
 DRY_OFF, DRY_UNIFORM, DRY_PER_SLOT = 0, 1, 2
 WET_OFF, WET_UNIFORM, WET_PER_SLOT, WET_PRECIP_FIELD = 0, 1, 2, 3
 LOSS_NONE, LOSS_MIN_MASS = 0, 1
-KERNEL_STEP, KERNEL_GRID, KERNEL_COMPACT, KERNEL_SORT, KERNEL_PBL = 0, 1, 2, 3, 4
+KERNEL_STEP, KERNEL_GRID, KERNEL_COMPACT, KERNEL_SORT, KERNEL_PBL, KERNEL_OUTPUT = 0, 1, 2, 3, 4, 5
 
 # The reference's 96-byte Particle (particles/mod.rs:76-168) == fp_particle.
 PARTICLE_DTYPE = np.dtype(

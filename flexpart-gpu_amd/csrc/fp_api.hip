@@ -567,8 +567,11 @@ int fp_output_sample(fp_ctx *ctx)
     FP_CHECK(ctx, ctx->out_samples < 0xffffffffu, "too many samples in one output interval");
     if (int rc = output_alloc(ctx)) return rc;
     const uint64_t ncell = (uint64_t)ctx->desc.out_nx * ctx->desc.out_ny * ctx->desc.out_nz;
+    fp_ctx::Ev ev{};
+    prof_begin(ctx, ev, FP_KERNEL_OUTPUT);
     FP_HIP(ctx, fp::launch_output_sample(ctx->grid_counts, ctx->grid_mass, ctx->out_acc_count, ctx->out_acc_mass,
                                          ncell, ctx->stream));
+    prof_end(ctx, ev);
     ++ctx->out_samples;
     return FP_OK;
 }

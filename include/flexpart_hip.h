@@ -396,7 +396,8 @@ int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *ou
 #define FP_KERNEL_COMPACT 2
 #define FP_KERNEL_SORT 3
 #define FP_KERNEL_PBL 4     /* fp_pbl_from_surface */
-#define FP_KERNEL_COUNT 5
+#define FP_KERNEL_OUTPUT 5  /* fp_output_sample */
+#define FP_KERNEL_COUNT 6
 int fp_profile_enable(fp_ctx *ctx, int enable);
 /* Blocking: total milliseconds and launch count since the last reset. */
 int fp_profile_read(fp_ctx *ctx, int kernel, double *total_ms, uint64_t *launches);
