@@ -155,6 +155,13 @@ class PblOptions(C.Structure):
         super().__init__(**d)
 
 
+class VtError(C.Structure):
+    _fields_ = [("code", C.c_uint32), ("i", C.c_uint32), ("j", C.c_uint32), ("k", C.c_uint32)]
+
+
+VT_OK, VT_NONPOSITIVE_PRESSURE, VT_NONDECREASING_PRESSURE, VT_INVALID_THERMODYNAMICS, VT_NONMONOTONIC_HEIGHT = range(5)
+
+
 class OutputGeometry(C.Structure):
     """fp_output_geometry: south-west corner and cell size of the output grid [deg]."""
     _fields_ = [("xlon0", C.c_double), ("ylat0", C.c_double), ("dx", C.c_double), ("dy", C.c_double)]
@@ -199,6 +206,11 @@ ABI = {
     "fp_surface_upload_dev": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int64]),
     "fp_pbl_from_surface": (C.c_int, [_vp, C.POINTER(PblOptions), C.c_int64, C.c_int, _vp]),
     "fp_pbl_download": (C.c_int, [_vp, _f, _f, _f, _f]),
+    "fp_host_alloc": (C.c_void_p, [C.c_uint64]),
+    "fp_host_free": (C.c_int, [_vp]),
+    "fp_met_prefetch_bracket": (C.c_int, [_vp, _f, _f, _f, _f, _f, _f, _f]),
+    "fp_met_commit_prefetch": (C.c_int, [_vp]),
+    "fp_met_hybrid_to_height": (C.c_int, [_vp, _f, _f, _f, _f, _f, _f, _f, _f, C.POINTER(VtError)]),
     "fp_output_reset": (C.c_int, [_vp]),
     "fp_output_sample": (C.c_int, [_vp]),
     "fp_output_finalize": (C.c_int, [_vp, C.POINTER(OutputGeometry), _u32, _f, _f, _u32]),
@@ -467,6 +479,48 @@ class GpuTimestep:
         self._chk(self._lib.fp_pbl_download(self._h, *[_fp(out[k]) for k in ("ustar", "wstar", "hmix", "oli")]))
         return out
 
+    # -- met ingest
+    def prefetch_bracket(self, u0, v0, w0, u1, v1, w1, level_heights):
+        """Start copying the next bracket on the copy stream (fp_met_prefetch_bracket).
+        The arrays are kept referenced until commit_prefetch; pinned_array()
+        memory makes the copy overlap the steps."""
+        n = int(np.prod(self.met_shape))
+        arrs = [_f32(a, n) for a in (u0, v0, w0, u1, v1, w1)]
+        lh = _f32(level_heights, self.met_shape[2])
+        self._chk(self._lib.fp_met_prefetch_bracket(self._h, *[_fp(a) for a in arrs], _fp(lh)))
+        self._prefetch_keep = arrs
+
+    def commit_prefetch(self):
+        """Switch to the prefetched bracket (fp_met_commit_prefetch)."""
+        self._chk(self._lib.fp_met_commit_prefetch(self._h))
+        self._prefetch_keep = None
+
+    def hybrid_to_height(self, ps, temperature, ak_pa, bk, humidity=None, fields: bool = False) -> dict:
+        """transform_hybrid_sigma_to_height on the device (fp_met_hybrid_to_height).
+        Returns mean_heights (the level heights) and, with fields=True, pressure
+        and height per column; raises FpError (err attribute: VtError) on a
+        transform error."""
+        nx, ny, nz = self.met_shape
+        p = _f32(ps, nx * ny)
+        tt = _f32(temperature, nx * ny * nz)
+        q = None if humidity is None else _f32(humidity, nx * ny * nz)
+        a, b = _f32(ak_pa, nz), _f32(bk, nz)
+        mean = np.empty(nz, np.float32)
+        pres = np.empty(nx * ny * nz, np.float32) if fields else None
+        hgt = np.empty(nx * ny * nz, np.float32) if fields else None
+        err = VtError()
+        rc = self._lib.fp_met_hybrid_to_height(self._h, _fp(p), _fp(tt), _fp(q), _fp(a), _fp(b), _fp(mean),
+                                               _fp(pres), _fp(hgt), C.byref(err))
+        if rc != FP_OK:
+            e = FpError(rc, self._lib.fp_last_error(self._h).decode())
+            e.err = err
+            raise e
+        out = {"mean_heights": mean}
+        if fields:
+            out["pressure"] = pres.reshape(nx, ny, nz)
+            out["height"] = hgt.reshape(nx, ny, nz)
+        return out
+
     # -- concentration output intervals
     def output_reset(self):
         self._chk(self._lib.fp_output_reset(self._h))
@@ -634,6 +688,23 @@ class GpuTimestep:
         return {"block_size": r.block_size, "particles_per_thread": r.particles_per_thread,
                 "sort_interval": r.sort_interval, "from_cache": bool(r.from_cache),
                 "step_ms": r.step_ms, "step_ms_with_sort": r.step_ms_with_sort}
+
+
+def pinned_array(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in pinned host memory from fp_host_alloc (freed when the
+    array is garbage-collected)."""
+    import weakref
+    lib = library()
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape))
+    nbytes = max(n * dt.itemsize, 1)
+    ptr = lib.fp_host_alloc(nbytes)
+    if not ptr:
+        raise MemoryError(f"fp_host_alloc({nbytes}) failed")
+    buf = (C.c_char * nbytes).from_address(ptr)
+    arr = np.frombuffer(buf, dtype=dt, count=n).reshape(shape)
+    weakref.finalize(buf, lib.fp_host_free, C.c_void_p(ptr))
+    return arr
 
 
 def write_netcdf(path: str, shape, count, mass_kg, time_s: int, species_index: int = 0, dry_kg_m2=None,

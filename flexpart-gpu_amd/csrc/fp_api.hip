@@ -114,15 +114,22 @@ void prof_end(fp_ctx *ctx, fp_ctx::Ev &ev)
     ctx->events.push_back(ev);
 }
 
-int validate_levels(fp_ctx *ctx, const float *lh)
+int check_levels(fp_ctx *ctx, const float *lh)
 {
     const uint32_t nz = ctx->desc.met_nz;
     for (uint32_t k = 0; k < nz; ++k)
         FP_CHECK(ctx, std::isfinite(lh[k]), "level heights must be finite");
-    const bool metres = lh[nz - 1] > 0.0f;
-    if (metres)
+    if (lh[nz - 1] > 0.0f)
         for (uint32_t k = 1; k < nz; ++k)
             FP_CHECK(ctx, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
+    return FP_OK;
+}
+
+int validate_levels(fp_ctx *ctx, const float *lh)
+{
+    if (int rc = check_levels(ctx, lh)) return rc;
+    const uint32_t nz = ctx->desc.met_nz;
+    const bool metres = lh[nz - 1] > 0.0f;
     ctx->has_heights = metres;
     ctx->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
     ctx->level_heights_host.assign(lh, lh + nz);
@@ -138,7 +145,10 @@ void destroy(fp_ctx *ctx)
     fr(ctx->met); fr(ctx->level_heights); fr(ctx->met_stage);
     fr(ctx->pbl); fr(ctx->precip); fr(ctx->pbl_stage); fr(ctx->surf[0]); fr(ctx->surf[1]);
     fr(ctx->out_acc_count); fr(ctx->out_acc_mass); fr(ctx->out_count); fr(ctx->out_mass); fr(ctx->out_conc);
-    fr(ctx->out_vol);
+    fr(ctx->out_vol); fr(ctx->prefetch_stage); fr(ctx->vt_buf); fr(ctx->vt_err);
+    if (ctx->copy_stream) { (void)hipStreamSynchronize(ctx->copy_stream); (void)hipStreamDestroy(ctx->copy_stream); }
+    if (ctx->prefetch_done) (void)hipEventDestroy(ctx->prefetch_done);
+    if (ctx->stage_free) (void)hipEventDestroy(ctx->stage_free);
     fr(ctx->dry_vd); fr(ctx->wet_lambda); fr(ctx->wet_fraction);
     fr(ctx->grid_counts); fr(ctx->grid_mass);
     fr(ctx->aos_stage); fr(ctx->slot_stage);
@@ -438,6 +448,105 @@ int fp_met_upload_bracket(fp_ctx *ctx, const float *u0, const float *v0, const f
     FP_CHECK(ctx, ctx && u0 && v0 && w0 && u1 && v1 && w1 && level_heights, "null argument");
     const float *f[6] = {u0, v0, w0, u1, v1, w1};
     return upload_met_common(ctx, f, hipMemcpyHostToDevice, level_heights);
+}
+
+void *fp_host_alloc(uint64_t bytes)
+{
+    void *p = nullptr;
+    if (bytes == 0 || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+int fp_host_free(void *p)
+{
+    if (!p) return FP_OK;
+    if (hipHostFree(p) != hipSuccess) return fail(nullptr, FP_E_DEVICE, "hipHostFree failed");
+    return FP_OK;
+}
+
+int fp_met_prefetch_bracket(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                            const float *u1, const float *v1, const float *w1,
+                            const float *level_heights)
+{
+    FP_CHECK(ctx, ctx && u0 && v0 && w0 && u1 && v1 && w1 && level_heights, "null argument");
+    if (ctx->prefetch_pending) return fail(ctx, FP_E_STATE, "a prefetched bracket is pending (fp_met_commit_prefetch)");
+    if (int rc = check_levels(ctx, level_heights)) return rc;
+    const uint64_t npts = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny * ctx->desc.met_nz;
+    if (!ctx->copy_stream) {
+        FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        FP_HIP(ctx, hipEventCreateWithFlags(&ctx->prefetch_done, hipEventDisableTiming));
+        FP_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_free, hipEventDisableTiming));
+    }
+    if (!ctx->prefetch_stage) FP_HIP(ctx, dalloc(&ctx->prefetch_stage, npts * 6));
+    // the previous commit's packing must have read the stage before it is overwritten
+    if (ctx->stage_used) FP_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->stage_free, 0));
+    const float *f[6] = {u0, v0, w0, u1, v1, w1};
+    for (int k = 0; k < 6; ++k)
+        FP_HIP(ctx, hipMemcpyAsync(ctx->prefetch_stage + k * npts, f[k], npts * 4, hipMemcpyHostToDevice,
+                                   ctx->copy_stream));
+    FP_HIP(ctx, hipEventRecord(ctx->prefetch_done, ctx->copy_stream));
+    ctx->prefetch_lh.assign(level_heights, level_heights + ctx->desc.met_nz);
+    ctx->prefetch_pending = true;
+    return FP_OK;
+}
+
+int fp_met_commit_prefetch(fp_ctx *ctx)
+{
+    FP_CHECK(ctx, ctx, "null context");
+    if (!ctx->prefetch_pending) return fail(ctx, FP_E_STATE, "no prefetched bracket (fp_met_prefetch_bracket)");
+    const uint64_t npts = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny * ctx->desc.met_nz;
+    FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->prefetch_done, 0));
+    if (int rc = validate_levels(ctx, ctx->prefetch_lh.data())) return rc;
+    FP_HIP(ctx, hipMemcpyAsync(ctx->level_heights, ctx->level_heights_host.data(), ctx->desc.met_nz * 4,
+                               hipMemcpyHostToDevice, ctx->stream));
+    const float *s = ctx->prefetch_stage;
+    FP_HIP(ctx, fp::launch_pack_met(s, s + npts, s + 2 * npts, s + 3 * npts, s + 4 * npts, s + 5 * npts, ctx->met,
+                                    (uint32_t)npts, ctx->stream));
+    FP_HIP(ctx, hipEventRecord(ctx->stage_free, ctx->stream));
+    ctx->stage_used = true;
+    ctx->prefetch_pending = false;
+    ctx->met_ready = true;
+    return FP_OK;
+}
+
+int fp_met_hybrid_to_height(fp_ctx *ctx, const float *ps, const float *temperature, const float *humidity,
+                            const float *ak_pa, const float *bk, float *mean_heights, float *pressure_out,
+                            float *height_out, fp_vt_error *err)
+{
+    FP_CHECK(ctx, ctx && ps && temperature && ak_pa && bk && mean_heights, "null argument");
+    if (err) *err = fp_vt_error{FP_VT_OK, 0, 0, 0};
+    const uint32_t ny = ctx->desc.met_ny, nz = ctx->desc.met_nz;
+    const uint64_t ncol = (uint64_t)ctx->desc.met_nx * ny, ncell = ncol * nz;
+    if (!ctx->vt_buf) {
+        FP_HIP(ctx, dalloc(&ctx->vt_buf, ncol + 4 * ncell + 3 * (uint64_t)nz));
+        FP_HIP(ctx, dalloc(&ctx->vt_err, 2));
+    }
+    float *d_ps = ctx->vt_buf, *d_t = d_ps + ncol, *d_q = d_t + ncell, *d_p = d_q + ncell, *d_z = d_p + ncell;
+    float *d_ak = d_z + ncell, *d_bk = d_ak + nz, *d_mean = d_bk + nz;
+    FP_HIP(ctx, hipMemcpyAsync(d_ps, ps, ncol * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(d_t, temperature, ncell * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (humidity) FP_HIP(ctx, hipMemcpyAsync(d_q, humidity, ncell * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(d_ak, ak_pa, nz * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(d_bk, bk, nz * 4, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemsetAsync(ctx->vt_err, 0xff, 16, ctx->stream));
+    FP_HIP(ctx, fp::launch_vertical_transform(d_ps, d_t, humidity ? d_q : nullptr, d_ak, d_bk, d_p, d_z, d_mean,
+                                              ctx->vt_err, (uint32_t)ncol, nz, ctx->stream));
+    unsigned long long keys[2];
+    FP_HIP(ctx, hipMemcpyAsync(keys, ctx->vt_err, 16, hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(mean_heights, d_mean, nz * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (pressure_out) FP_HIP(ctx, hipMemcpyAsync(pressure_out, d_p, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (height_out) FP_HIP(ctx, hipMemcpyAsync(height_out, d_z, ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long key = keys[0] != ~0ull ? keys[0] : keys[1];
+    if (key != ~0ull) {
+        const uint32_t c = (uint32_t)(key >> 32), k = (uint32_t)((key >> 8) & 0xffffffu), code = (uint32_t)(key & 0xffu);
+        if (err) *err = fp_vt_error{code, c / ny, c % ny, k};
+        static const char *what[5] = {"", "non-positive pressure", "pressure must decrease with level index",
+                                      "invalid thermodynamic state", "geometric heights must increase with level index"};
+        return fail(ctx, FP_E_INVALID, std::string(what[code < 5 ? code : 0]) + " at (i=" + std::to_string(c / ny) +
+                                           ", j=" + std::to_string(c % ny) + ", k=" + std::to_string(k) + ")");
+    }
+    return FP_OK;
 }
 
 int fp_met_upload_bracket_dev(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,

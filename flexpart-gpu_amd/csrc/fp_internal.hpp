@@ -194,6 +194,17 @@ struct fp_ctx {
     void *sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
 
+    // asynchronous bracket prefetch
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t prefetch_done = nullptr, stage_free = nullptr;
+    float *prefetch_stage = nullptr;  // 6 * npts
+    bool prefetch_pending = false, stage_used = false;
+    std::vector<float> prefetch_lh;
+
+    // hybrid-level to height transform buffers
+    float *vt_buf = nullptr;  // ps (ncol) | T | q | p | z (ncell each) | ak | bk | mean (nz each)
+    unsigned long long *vt_err = nullptr;
+
     // concentration output: interval sums and finalize buffers
     unsigned long long *out_acc_count = nullptr, *out_acc_mass = nullptr;
     uint32_t out_samples = 0;
@@ -248,6 +259,10 @@ hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *re
 // PBL records from the two surface bracket ends (fp_surface.hip).
 hipError_t launch_pbl_surface(const float *const s0[FP_SURF_COUNT], const float *const s1[FP_SURF_COUNT], float alpha,
                               const fp_pbl_options &o, float4 *pbl, float *pass, uint32_t n, hipStream_t st);
+// Hybrid-level to height transform (fp_met.hip); err: two u64 atomic-min keys.
+hipError_t launch_vertical_transform(const float *ps, const float *t, const float *q, const float *ak,
+                                     const float *bk, float *pressure, float *height, float *mean,
+                                     unsigned long long *err, uint32_t ncol, uint32_t nz, hipStream_t st);
 // Concentration output (fp_output.hip).
 hipError_t launch_output_sample(const uint32_t *counts, const uint32_t *mass, unsigned long long *acc_count,
                                 unsigned long long *acc_mass, uint64_t n, hipStream_t st);

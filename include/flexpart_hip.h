@@ -160,6 +160,49 @@ int fp_met_upload_bracket_dev(fp_ctx *ctx, const float *u0, const float *v0, con
  * met_nx x met_ny, index i*ny+j; precip [mm/h] may be NULL. */
 int fp_pbl_upload(fp_ctx *ctx, const float *ustar, const float *wstar, const float *hmix,
                   const float *oli, const float *precip);
+/* ---- met ingest on the device ---------------------------------------- */
+
+/* Pinned host memory (hipHostMalloc) for met fields: H2D copies from it run
+ * on the DMA engines without a host-side staging copy.  NULL on failure. */
+void *fp_host_alloc(uint64_t bytes);
+int fp_host_free(void *p);
+
+/* Asynchronous bracket upload: the six fields of the next met bracket are
+ * copied on the context's copy stream while fp_step keeps using the current
+ * bracket; fp_met_commit_prefetch then orders the context's stream after the
+ * copies and packs the new bracket (as fp_met_upload_bracket does) without
+ * blocking the host.  The replacement for the reference's prefetch pipeline
+ * (io/grib2_async.rs:49-96 decoding on a background thread, then the
+ * DualWindBuffers uploads of timeloop.rs:1378-1402).  The host arrays must
+ * stay untouched until the commit returns; use fp_host_alloc memory for a
+ * transfer that overlaps the steps.  One prefetch may be pending at a time. */
+int fp_met_prefetch_bracket(fp_ctx *ctx, const float *u0, const float *v0, const float *w0,
+                            const float *u1, const float *v1, const float *w1,
+                            const float *level_heights);
+int fp_met_commit_prefetch(fp_ctx *ctx);
+
+/* transform_hybrid_sigma_to_height (io/vertical_transform.rs:144-212) on the
+ * met grid: p = ak + bk ps per level, virtual temperature, hypsometric layer
+ * thicknesses into geometric heights per column, and the horizontal mean
+ * height profile (the grid's level heights, for fp_met_upload_bracket).
+ * ps: met_nx*met_ny; temperature, humidity (NULL = dry): met_nx*met_ny*met_nz,
+ * index ((i*ny)+j)*nz+k; ak_pa, bk: met_nz.  mean_heights: met_nz floats
+ * (host); pressure_out / height_out: optional host arrays of the per-column
+ * fields.  A transform error returns FP_E_INVALID with *err (may be NULL)
+ * holding the reference's error kind and its (i, j, k). */
+#define FP_VT_OK 0u
+#define FP_VT_NONPOSITIVE_PRESSURE 1u
+#define FP_VT_NONDECREASING_PRESSURE 2u
+#define FP_VT_INVALID_THERMODYNAMICS 3u
+#define FP_VT_NONMONOTONIC_HEIGHT 4u
+typedef struct fp_vt_error {
+    uint32_t code;      /* FP_VT_* */
+    uint32_t i, j, k;
+} fp_vt_error;
+int fp_met_hybrid_to_height(fp_ctx *ctx, const float *ps, const float *temperature, const float *humidity,
+                            const float *ak_pa, const float *bk, float *mean_heights, float *pressure_out,
+                            float *height_out, fp_vt_error *err);
+
 /* ---- PBL from surface fields on the device ---------------------------- */
 
 /* The ten surface fields the PBL diagnostics read, in the order of the

@@ -1070,3 +1070,84 @@ void orc_output_finalize(uint64_t ncell, const uint64_t *acc_count, const uint64
         }
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Hybrid sigma-pressure to height: io/vertical_transform.rs                */
+/* ---------------------------------------------------------------------- */
+
+/* virtual_temperature_k (vertical_transform.rs:272-292) */
+static int vt_virtual_temperature(float t, float q, float *tv)
+{
+    if (!(t == t && fabsf(t) <= FLT_MAX) || t <= 0.0f || !(q == q && fabsf(q) <= FLT_MAX)) return 0;
+    *tv = t * (1.0f + 0.608f * o_max(q, 0.0f));
+    return 1;
+}
+
+/* hypsometric_layer_thickness_m (vertical_transform.rs:294-313) */
+static float vt_thickness(float p_prev, float p_cur, float tv_prev, float tv_cur)
+{
+    const float scale = 287.05f / 9.81f; /* R_AIR / GA */
+    const float pr_ln = O_LOG(p_prev / p_cur);
+    const float dtv = tv_cur - tv_prev;
+    const float tvr_ln = O_LOG(tv_cur / tv_prev);
+    if (fabsf(dtv) > 0.2f && fabsf(tvr_ln) > FLT_EPSILON) return scale * pr_ln * dtv / tvr_ln;
+    return scale * pr_ln * tv_cur;
+}
+
+int orc_hybrid_to_height(uint32_t nx, uint32_t ny, uint32_t nz, const float *ps, const float *t, const float *q,
+                         const float *ak, const float *bk, float *pressure, float *height, float *mean,
+                         uint32_t err_ijk[3])
+{
+    /* compute_hybrid_level_pressure_pa */
+    for (uint32_t i = 0; i < nx; ++i)
+        for (uint32_t j = 0; j < ny; ++j)
+            for (uint32_t k = 0; k < nz; ++k) {
+                const uint64_t c = ((uint64_t)i * ny + j) * nz + k;
+                const float p = ak[k] + bk[k] * ps[(uint64_t)i * ny + j];
+                if (!(p == p && fabsf(p) <= FLT_MAX) || p <= 0.0f) {
+                    err_ijk[0] = i; err_ijk[1] = j; err_ijk[2] = k;
+                    return 1;
+                }
+                pressure[c] = p;
+            }
+    for (uint32_t i = 0; i < nx; ++i)
+        for (uint32_t j = 0; j < ny; ++j) {
+            const uint64_t base = ((uint64_t)i * ny + j) * nz;
+            float p_prev = pressure[base], tv_prev;
+            height[base] = 0.0f;
+            if (!vt_virtual_temperature(t[base], q ? q[base] : 0.0f, &tv_prev)) {
+                err_ijk[0] = i; err_ijk[1] = j; err_ijk[2] = 0;
+                return 3;
+            }
+            for (uint32_t k = 1; k < nz; ++k) {
+                const float p = pressure[base + k];
+                if (p >= p_prev) {
+                    err_ijk[0] = i; err_ijk[1] = j; err_ijk[2] = k;
+                    return 2;
+                }
+                float tv;
+                if (!vt_virtual_temperature(t[base + k], q ? q[base + k] : 0.0f, &tv)) {
+                    err_ijk[0] = i; err_ijk[1] = j; err_ijk[2] = k;
+                    return 3;
+                }
+                const float z_prev = height[base + k - 1];
+                const float z = z_prev + vt_thickness(p_prev, p, tv_prev, tv);
+                if (!(z == z && fabsf(z) <= FLT_MAX) || z <= z_prev) {
+                    err_ijk[0] = i; err_ijk[1] = j; err_ijk[2] = k;
+                    return 4;
+                }
+                height[base + k] = z;
+                p_prev = p;
+                tv_prev = tv;
+            }
+        }
+    /* horizontal_mean_height_profile_m */
+    const float count = (float)((uint64_t)nx * ny);
+    for (uint32_t k = 0; k < nz; ++k) {
+        float s = 0.0f;
+        for (uint32_t i = 0; i < nx; ++i)
+            for (uint32_t j = 0; j < ny; ++j) s += height[((uint64_t)i * ny + j) * nz + k];
+        mean[k] = s / count;
+    }
+    return 0;
+}

@@ -149,6 +149,15 @@ typedef struct {
 void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], const float *const s1[ORC_SURF_COUNT],
                           float alpha, const orc_pbl_options *o, float *const out[7]);
 
+/* transform_hybrid_sigma_to_height (io/vertical_transform.rs:144-212) and
+ * compute_hybrid_level_pressure_pa (:106-137), in the reference's loop order;
+ * the mean profile is its sequential f32 sum (:315-334).  Returns 0 or the
+ * error kind (1 non-positive pressure, 2 non-decreasing pressure, 3 invalid
+ * thermodynamics, 4 non-monotonic height) with err_ijk = (i, j, k). */
+int orc_hybrid_to_height(uint32_t nx, uint32_t ny, uint32_t nz, const float *ps, const float *t, const float *q,
+                         const float *ak, const float *bk, float *pressure, float *height, float *mean,
+                         uint32_t err_ijk[3]);
+
 /* Concentration output.  Volumes of the output cells per row iy and level k
  * (docs/science/concentration-gridding.md:35-41 with dx_meters_at_latitude,
  * coords/mod.rs:156-158): vol[iy*nz+k]. */

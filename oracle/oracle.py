@@ -460,3 +460,25 @@ def output_finalize(acc_count, acc_mass, n_samples: int, mass_scale: float, shap
     if conc is not None:
         out["conc_kg_m3"] = conc
     return out
+
+
+def hybrid_to_height(ps, t, ak, bk, q=None, variant="fpm"):
+    """transform_hybrid_sigma_to_height (orc_hybrid_to_height): ps (nx, ny),
+    t and q (nx, ny, nz), ak and bk (nz).  Returns (code, (i, j, k), pressure,
+    height, mean)."""
+    ps = np.ascontiguousarray(np.asarray(ps, np.float32))
+    t = np.ascontiguousarray(np.asarray(t, np.float32))
+    nx, ny, nz = t.shape
+    qa = None if q is None else np.ascontiguousarray(np.asarray(q, np.float32))
+    a = np.ascontiguousarray(np.asarray(ak, np.float32))
+    b = np.ascontiguousarray(np.asarray(bk, np.float32))
+    pressure = np.zeros((nx, ny, nz), np.float32)
+    height = np.zeros((nx, ny, nz), np.float32)
+    mean = np.zeros(nz, np.float32)
+    ijk = (C.c_uint32 * 3)()
+    fn = lib(variant).orc_hybrid_to_height
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_uint32] * 3 + [C.c_void_p] * 8 + [C.c_void_p]
+    code = fn(nx, ny, nz, ps.ctypes.data, t.ctypes.data, None if qa is None else qa.ctypes.data, a.ctypes.data,
+              b.ctypes.data, pressure.ctypes.data, height.ctypes.data, mean.ctypes.data, ijk)
+    return code, tuple(ijk), pressure, height, mean
