@@ -6,11 +6,9 @@
 // T (1 + 0.608 max(q, 0)), hypsometric layer thicknesses (the
 // |Tv - Tv_old| > 0.2 switch of verttransform_ecmwf.f90) accumulated into
 // geometric heights, then the horizontal mean height profile that becomes the
-// grid's level heights.  Here one thread per column runs the level loop (the
-// column's levels are contiguous, so a wave's loads walk 64 neighbouring
-// columns), and the mean profile is the reference's sequential f32 sum in
-// (i, j) order with one lane per level: lane k adds height[c*nz + k], so the
-// lanes of a wave read one column's levels in one coalesced access per c.
+// grid's level heights.  Here one thread per column runs the level loop, and
+// the mean profile is the reference's sequential f32 sum in (i, j) order with
+// one lane per level, fed from LDS tiles that other waves stream in.
 // Errors keep the reference's precedence: any pressure error (first in
 // (i, j, k) order) before any error of the height loop (first column in
 // (i, j) order, its first failing level).
@@ -114,26 +112,73 @@ __global__ void __launch_bounds__(kVtBlock) vt_column_kernel(VtArgs a)
     }
 }
 
-// horizontal_mean_height_profile_m (vertical_transform.rs:315-334): lane k of
-// the launch sums level k over the columns in (i, j) order in f32, then
-// divides by (f32)(nx*ny).  Four partial loads are in flight per step; the
-// additions stay in order.
-__global__ void __launch_bounds__(64) vt_mean_kernel(const float *height, uint32_t ncol, uint32_t nz, float *mean)
+// horizontal_mean_height_profile_m (vertical_transform.rs:315-334): the sum of
+// level k over the columns in (i, j) order in f32, divided by (f32)(nx*ny).
+// One block: lane k of the summing waves owns level k and adds the columns in
+// order from an LDS tile (lanes read consecutive words of one column, no bank
+// conflicts).  The other waves keep the next tile in registers: each iteration
+// they store it to the free LDS buffer and issue the loads of the tile after,
+// so a tile's HBM latency overlaps the summation of the previous one.
+constexpr uint32_t kMeanBlock = 512;
+constexpr uint32_t kMeanTileFloats = 8192;  // per LDS buffer (32 KB)
+constexpr uint32_t kMeanPer = 32;           // tile floats per loader thread (>= 8192 / 256)
+
+__global__ void __launch_bounds__(kMeanBlock) vt_mean_kernel(const float *height, uint32_t ncol, uint32_t nz,
+                                                            float *mean)
 {
-    const uint32_t k = blockIdx.x * 64u + threadIdx.x;
-    if (k >= nz) return;
-    float s = 0.0f;
-    uint32_t c = 0;
-    for (; c + 4u <= ncol; c += 4u) {
-        const float h0 = height[(uint64_t)c * nz + k], h1 = height[(uint64_t)(c + 1u) * nz + k];
-        const float h2 = height[(uint64_t)(c + 2u) * nz + k], h3 = height[(uint64_t)(c + 3u) * nz + k];
-        s += h0;
-        s += h1;
-        s += h2;
-        s += h3;
+    __shared__ float buf[2][kMeanTileFloats];
+    const uint32_t tile = kMeanTileFloats / nz;  // columns per tile
+    const uint32_t ntiles = (ncol + tile - 1) / tile;
+    const uint32_t sum_threads = (nz + kWave - 1) / kWave * kWave;
+    const uint32_t tid = threadIdx.x;
+    const bool summer = tid < sum_threads;
+    const uint32_t lid = tid - sum_threads, nload = kMeanBlock - sum_threads;
+    float r[kMeanPer];
+    auto tile_floats = [&](uint32_t tix) { return min(tile, ncol - tix * tile) * nz; };
+    auto fetch = [&](uint32_t tix) {  // loaders: tile tix into registers
+        const uint64_t base = (uint64_t)tix * tile * nz;
+        const uint32_t n = tile_floats(tix);
+#pragma unroll
+        for (uint32_t q = 0; q < kMeanPer; ++q) {
+            const uint32_t e = lid + q * nload;
+            if (e < n) r[q] = height[base + e];
+        }
+    };
+    auto put = [&](uint32_t tix, float *dst) {  // loaders: registers into LDS
+        const uint32_t n = tile_floats(tix);
+#pragma unroll
+        for (uint32_t q = 0; q < kMeanPer; ++q) {
+            const uint32_t e = lid + q * nload;
+            if (e < n) dst[e] = r[q];
+        }
+    };
+    {  // tile 0 by every thread
+        const uint32_t n = tile_floats(0);
+        for (uint32_t e = tid; e < n; e += kMeanBlock) buf[0][e] = height[e];
     }
-    for (; c < ncol; ++c) s += height[(uint64_t)c * nz + k];
-    mean[k] = s / (float)ncol;
+    if (!summer && ntiles > 1) fetch(1);
+    __syncthreads();
+    float s = 0.0f;
+    for (uint32_t tix = 0; tix < ntiles; ++tix) {
+        if (!summer) {
+            if (tix + 1 < ntiles) put(tix + 1, buf[(tix + 1) & 1]);
+            if (tix + 2 < ntiles) fetch(tix + 2);
+        } else if (tid < nz) {
+            const float *b = buf[tix & 1] + tid;
+            const uint32_t cnt = min(tile, ncol - tix * tile);
+            uint32_t c = 0;
+            for (; c + 16u <= cnt; c += 16u) {  // 16 LDS reads in flight, then the adds in order
+                float h[16];
+#pragma unroll
+                for (uint32_t u = 0; u < 16u; ++u) h[u] = b[(c + u) * nz];
+#pragma unroll
+                for (uint32_t u = 0; u < 16u; ++u) s += h[u];
+            }
+            for (; c < cnt; ++c) s += b[c * nz];
+        }
+        __syncthreads();
+    }
+    if (tid < nz) mean[tid] = s / (float)ncol;
 }
 
 }  // namespace
@@ -145,7 +190,7 @@ hipError_t launch_vertical_transform(const float *ps, const float *t, const floa
     VtArgs a{ps, t, q, ak, bk, pressure, height, err, err + 1, ncol, nz};
     vt_column_kernel<<<(ncol + kVtBlock - 1) / kVtBlock, kVtBlock, 0, st>>>(a);
     if (hipError_t e = hipGetLastError()) return e;
-    vt_mean_kernel<<<(nz + 63u) / 64u, 64, 0, st>>>(height, ncol, nz, mean);
+    vt_mean_kernel<<<1, kMeanBlock, 0, st>>>(height, ncol, nz, mean);
     return hipGetLastError();
 }
 
