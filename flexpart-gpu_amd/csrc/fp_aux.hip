@@ -83,11 +83,18 @@ __global__ void __launch_bounds__(kT) aos_to_soa_tiled(const uint32_t *aos, uint
 //                      copies them to offset + rank; slots at or beyond the
 //                      total receive a deactivated copy of themselves
 // The survivor order is ascending slot order, as in compaction.wgsl.
+//
+// The scatter is a 24-stream copy and HBM-bound: a block first ranks all of
+// its kRows rows (one barrier), then copies field by field, issuing the loads
+// of kFieldGroup fields x kRows rows before their stores, so each wave keeps
+// 16 independent loads in flight instead of waiting on a barrier per row.
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kRows = 4;
+constexpr uint32_t kRows = 8;
 constexpr uint32_t kTile = kBlock * kRows;
 constexpr uint32_t kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kFieldGroup = 2;
+static_assert(F_COUNT % kFieldGroup == 0, "field groups");
 
 __device__ __forceinline__ uint32_t active_at(const Soa &s, uint64_t i, uint32_t n)
 {
@@ -138,48 +145,65 @@ __global__ void __launch_bounds__(1024) scan_kernel(uint32_t *tile_counts, uint3
     if (t == 1023u) *total = part[1023];
 }
 
-__device__ __forceinline__ void copy_slot(const Soa &in, const Soa &out, uint64_t src, uint64_t dst,
-                                          bool deactivate)
-{
-#pragma unroll
-    for (uint32_t f = 0; f < F_COUNT; ++f) {
-        uint32_t v = in.u(f)[src];
-        if (f == F_FLAGS && deactivate) v &= ~FLAG_ACTIVE;
-        out.u(f)[dst] = v;
-    }
-}
-
 __global__ void __launch_bounds__(kBlock) scatter_kernel(Soa in, Soa out, uint32_t n,
                                                          const uint32_t *tile_offsets,
                                                          const uint32_t *total_p,
                                                          uint32_t *survivor_map)
 {
-    __shared__ uint32_t wcount[kWavesPerBlock];
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
-    const uint32_t total = *total_p;
+    __shared__ uint32_t wcount[kRows][kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + threadIdx.x;
     const uint32_t wave = threadIdx.x / kWave, lane = __lane_id();
-    uint32_t running = tile_offsets[blockIdx.x];
+    const uint64_t below_mask = (1ull << lane) - 1ull;
+    // ranks within each row's wave, then the row offsets of the waves
+    uint32_t act = 0, dst[kRows];
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) act |= active_at(in, base + r * kBlock, n) << r;
+#pragma unroll
     for (uint32_t r = 0; r < kRows; ++r) {
-        const uint64_t i = base + r * kBlock + threadIdx.x;
-        const uint32_t a = active_at(in, i, n);
-        const uint64_t bal = __ballot(a != 0u);
-        const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wcount[wave] = (uint32_t)__popcll(bal);
-        __syncthreads();
+        const uint64_t bal = __ballot((act >> r) & 1u);
+        dst[r] = (uint32_t)__popcll(bal & below_mask);
+        if (lane == 0) wcount[r][wave] = (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    const uint32_t total = *total_p;
+    uint32_t running = tile_offsets[blockIdx.x];
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) {
         uint32_t woff = 0, rowtot = 0;
+#pragma unroll
         for (uint32_t w = 0; w < kWavesPerBlock; ++w) {
-            if (w < wave) woff += wcount[w];
-            rowtot += wcount[w];
+            const uint32_t c = wcount[r][w];
+            woff += w < wave ? c : 0u;
+            rowtot += c;
         }
-        __syncthreads();
-        if (a) {
-            const uint32_t dst = running + woff + below;
-            copy_slot(in, out, i, dst, false);
-            survivor_map[dst] = (uint32_t)i;
-        }
-        if (i < n && i >= total) copy_slot(in, out, i, i, true);
+        dst[r] += running + woff;
         running += rowtot;
     }
+    // survivors to offset + rank; slots at or beyond the total (whether
+    // active or not) also keep a deactivated copy of themselves
+#pragma unroll
+    for (uint32_t f0 = 0; f0 < F_COUNT; f0 += kFieldGroup) {
+        uint32_t v[kFieldGroup][kRows];
+#pragma unroll
+        for (uint32_t g = 0; g < kFieldGroup; ++g)
+#pragma unroll
+            for (uint32_t r = 0; r < kRows; ++r) {
+                const uint64_t i = base + r * kBlock;
+                v[g][r] = i < n ? in.u(f0 + g)[i] : 0u;
+            }
+#pragma unroll
+        for (uint32_t g = 0; g < kFieldGroup; ++g)
+#pragma unroll
+            for (uint32_t r = 0; r < kRows; ++r) {
+                const uint64_t i = base + r * kBlock;
+                const uint32_t f = f0 + g;
+                if ((act >> r) & 1u) out.u(f)[dst[r]] = v[g][r];
+                if (i < n && i >= total) out.u(f)[i] = f == F_FLAGS ? (v[g][r] & ~FLAG_ACTIVE) : v[g][r];
+            }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r)
+        if ((act >> r) & 1u) survivor_map[dst[r]] = (uint32_t)(base + r * kBlock);
 }
 
 __global__ void philox_kernel(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0_in,
