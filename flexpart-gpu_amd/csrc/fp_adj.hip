@@ -1135,29 +1135,59 @@ int fp_adj_forward(fp_adj *e, const fp_step_params *p, const float *q, const flo
     const fp::ObsArgs oa = obs_args(e);
     e->step_n[0] = (uint32_t)e->n;
     for (int cc = 0; cc < 4; ++cc) e->step_ctr[0][cc] = p->counter[cc];
+    // With compaction after every step (timeloop.rs:1185-1197), the records
+    // are not moved by the compaction itself: step k + 1 reads survivor j of
+    // step k through the saved map (launch_step_gathered) and writes it to
+    // slot j, which is the state and the Philox slot a moved record would have.
+    // The gathered step writes the words a step changes; every other word of
+    // a record is zero here in both sets (sources start with zero words, the
+    // sets are zeroed at creation and no kernel writes those words), so the
+    // records are whole.  Only the last compaction moves the records, so that
+    // the final state is the compacted one.
     uint64_t moff = 0;
     for (uint32_t k = 0; k < e->d.n_steps; ++k) {
         const uint32_t nk = e->step_n[k];
-        const fp::StepArgs a = step_args(e, k, e->work);
-        A_HIP(e, fp::launch_step(a, 256, st));
+        const bool gathered = e->compact && k > 0;
+        fp::StepArgs a = step_args(e, k, gathered ? e->work2 : e->work);
+        if (gathered) {
+            a.gather_map = e->maps + e->map_off[k - 1u];
+            a.src = e->work;
+            A_HIP(e, fp::launch_step_gathered(a, false, st));
+            std::swap(e->work, e->work2);
+        } else {
+            A_HIP(e, fp::launch_step(a, 256, st));
+        }
+        // e->work: the state after step k (before its compaction)
         if (observed(e, k + 1u) && nk)
             fp::obs_deposit_kernel<<<fp::grid_for(nk), 256, 0, st>>>(e->work, nk, oa, e->rec_val);
         uint32_t next = nk;
+        const bool last = k + 1u == e->d.n_steps;
         if (e->compact) {
-            // compaction after the step (timeloop.rs:1185-1197); its survivor map is kept
             e->map_off[k] = moff;
-            A_HIP(e, fp::launch_compact(e->work, e->work2, nk, e->block_counts, e->maps + moff, e->active_dev, st));
+            if (last)
+                A_HIP(e, fp::launch_compact(e->work, e->work2, nk, e->block_counts, e->maps + moff, e->active_dev,
+                                            st));
+            else
+                A_HIP(e, fp::launch_survivor_map(e->work, nk, e->block_counts, e->maps + moff, e->active_dev, st));
             A_HIP(e, hipMemcpyAsync(e->active_host, e->active_dev, 4, hipMemcpyDeviceToHost, st));
             A_HIP(e, hipStreamSynchronize(st));
             next = *e->active_host;
             moff += next;
-            std::swap(e->work, e->work2);
+            if (last) std::swap(e->work, e->work2);
         }
         e->step_n[k + 1u] = next;
         fp::counter_advance(e->step_ctr[k].data(), fp::counter_per_step(nk, p->n_substeps),
                             e->step_ctr[k + 1u].data());
         const int j = ckpt_index(e, k + 1u);
-        if (j >= 0) A_HIP(e, copy_state(e, e->ckpt[j], e->work, next));
+        if (j >= 0) {
+            if (e->compact && !last) {
+                if (next)
+                    fp::gather_state_kernel<<<fp::grid_for(next), 256, 0, st>>>(e->work, e->ckpt[j],
+                                                                              e->maps + e->map_off[k], next);
+            } else {
+                A_HIP(e, copy_state(e, e->ckpt[j], e->work, next));
+            }
+        }
     }
     A_HIP(e, hipGetLastError());
     A_HIP(e, hipEventRecord(e->e1, st));
@@ -1211,11 +1241,12 @@ int fp_adj_backward(fp_adj *e, double *grad_q, double *grad_r, double *grad_c)
             const fp::Soa &prev = (k - 1u == k0) ? e->ckpt[seg] : e->tape[k - 2u - k0];
             const fp::Soa &cur = e->tape[k - 1u - k0];
             if (e->compact) {
-                A_HIP(e, copy_state(e, e->work2, prev, e->step_n[k - 1u]));
-                A_HIP(e, fp::launch_step(step_args(e, k - 1u, e->work2), 256, st));
-                if (e->step_n[k])
-                    fp::gather_state_kernel<<<fp::grid_for(e->step_n[k]), 256, 0, st>>>(
-                        e->work2, cur, e->maps + e->map_off[k - 1u], e->step_n[k]);
+                // only the survivors of step k - 1, each stepped at its own slot
+                fp::StepArgs a = step_args(e, k - 1u, cur);
+                a.n = e->step_n[k];
+                a.gather_map = e->maps + e->map_off[k - 1u];
+                a.src = prev;
+                A_HIP(e, fp::launch_step_gathered(a, true, st));
             } else {
                 A_HIP(e, copy_state(e, cur, prev, e->step_n[k - 1u]));
                 A_HIP(e, fp::launch_step(step_args(e, k - 1u, cur), 256, st));

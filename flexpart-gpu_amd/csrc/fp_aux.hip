@@ -145,6 +145,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(uint32_t *tile_counts, uint3
     if (t == 1023u) *total = part[1023];
 }
 
+// COPY = false writes the survivor map only (the records stay where they are).
+template <bool COPY>
 __global__ void __launch_bounds__(kBlock) scatter_kernel(Soa in, Soa out, uint32_t n,
                                                          const uint32_t *tile_offsets,
                                                          const uint32_t *total_p,
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(Soa in, Soa out, uint32
     // survivors to offset + rank; slots at or beyond the total (whether
     // active or not) also keep a deactivated copy of themselves
 #pragma unroll
-    for (uint32_t f0 = 0; f0 < F_COUNT; f0 += kFieldGroup) {
+    for (uint32_t f0 = 0; f0 < (COPY ? F_COUNT : 0u); f0 += kFieldGroup) {
         uint32_t v[kFieldGroup][kRows];
 #pragma unroll
         for (uint32_t g = 0; g < kFieldGroup; ++g)
@@ -292,7 +294,18 @@ hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *t
     if (ntiles == 0) return hipMemsetAsync(active_count, 0, sizeof(uint32_t), st);
     count_kernel<<<ntiles, kBlock, 0, st>>>(in, n, tile_counts);
     scan_kernel<<<1, 1024, 0, st>>>(tile_counts, ntiles, active_count);
-    scatter_kernel<<<ntiles, kBlock, 0, st>>>(in, out, n, tile_counts, active_count, survivor_map);
+    scatter_kernel<true><<<ntiles, kBlock, 0, st>>>(in, out, n, tile_counts, active_count, survivor_map);
+    return hipGetLastError();
+}
+
+hipError_t launch_survivor_map(const Soa &in, uint32_t n, uint32_t *tile_counts, uint32_t *survivor_map,
+                               uint32_t *active_count, hipStream_t st)
+{
+    const uint32_t ntiles = (uint32_t)compact_blocks(n);
+    if (ntiles == 0) return hipMemsetAsync(active_count, 0, sizeof(uint32_t), st);
+    count_kernel<<<ntiles, kBlock, 0, st>>>(in, n, tile_counts);
+    scan_kernel<<<1, 1024, 0, st>>>(tile_counts, ntiles, active_count);
+    scatter_kernel<false><<<ntiles, kBlock, 0, st>>>(in, in, n, tile_counts, active_count, survivor_map);
     return hipGetLastError();
 }
 

@@ -54,6 +54,9 @@ struct GridArgs {
     float outheights[FP_MAX_OUTPUT_LEVELS];
 };
 
+// Where a gathered step (launch_step_gathered) takes each particle's step.
+constexpr int kGatherNone = 0, kGatherToSlot = 1, kGatherFromSlot = 2;
+
 // Everything the fused step kernel reads, passed by value.
 struct StepArgs {
     Soa s;
@@ -86,6 +89,12 @@ struct StepArgs {
     GridArgs grid;
     uint32_t ppt;           // particles per thread
     double log_1em3;        // fpm_log_d(1e-3f): the log of the unstable profile's zeta floor
+    // gathered steps (launch_step_gathered): slot i of s receives the words
+    // the step writes of src slot gather_map[i] advanced by one step, i.e. the
+    // step of the state that a compaction with survivor map gather_map would
+    // produce; the other words of s are left as they are
+    const uint32_t *gather_map;
+    Soa src;
 };
 
 // Ensemble launch tables (device memory).
@@ -234,6 +243,9 @@ namespace fp {
 
 // Kernel launchers (fp_step.hip, fp_aux.hip).
 hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
+// one step of a.n slots through a.gather_map (block 256, a.ppt rows); the
+// particle steps at its destination slot, or at its source slot
+hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st);
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
                            uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st);
 hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
@@ -246,6 +258,9 @@ hipError_t launch_soa_to_aos(Soa s, fp_particle *aos, uint64_t n, hipStream_t st
 hipError_t launch_grid(const Soa &s, uint32_t n, const GridArgs &g, hipStream_t st);
 hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *block_counts,
                           uint32_t *survivor_map, uint32_t *active_count, hipStream_t st);
+// the survivor map and active count of launch_compact without moving the records
+hipError_t launch_survivor_map(const Soa &in, uint32_t n, uint32_t *block_counts, uint32_t *survivor_map,
+                               uint32_t *active_count, hipStream_t st);
 uint64_t compact_blocks(uint32_t n);
 hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2], uint64_t n,
                                 uint32_t *out, hipStream_t st);

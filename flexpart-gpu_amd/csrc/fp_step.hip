@@ -24,9 +24,21 @@
 
 namespace fp {
 
+// Words of a record that the step writes for an active particle.
+__device__ __forceinline__ bool stepped_field(uint32_t f, uint32_t n_species)
+{
+    return f <= F_FLAGS || (f >= F_VEL_U && f <= F_TURB_W) || f == F_CBT ||
+           (f >= F_MASS0 && f < F_MASS0 + n_species);
+}
+
 // One block's chunk of the step: slots first + r*BLOCK + threadIdx.x for
-// r < a.ppt, temporal weight alpha and Philox base counter ctr.
-template <uint32_t BLOCK, int NSUB>
+// r < a.ppt, temporal weight alpha and Philox base counter ctr.  With GATHER,
+// slot i of a.s receives the step of slot a.gather_map[i] of a.src, taken at
+// slot i (kGatherToSlot: the record was moved there by a compaction before the
+// step) or at its source slot (kGatherFromSlot: the record is moved there by a
+// compaction after the step) - the slot that selects its Philox counters and
+// per-slot forcing.
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone>
 __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, float alpha, U4 ctr,
                                            const float *lh, const float *oh, GridLds &gt)
 {
@@ -38,6 +50,12 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
     for (uint32_t r = 0; r < a.ppt; ++r) {
         const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
         const bool inb = i < a.n;
+        // the slot the state is read from
+        uint64_t g = i;
+        if (GATHER != kGatherNone && inb) g = a.gather_map[i];
+        const Soa sl = GATHER != kGatherNone ? a.src : s;
+        // the slot the particle occupies during the step
+        const uint64_t at = GATHER == kGatherFromSlot ? g : i;
         // The slot's whole hot state is requested at once (44 B, coalesced);
         // slots whose flags say inactive are not used further.
         uint32_t flags = 0;
@@ -45,20 +63,20 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         float px = 0.0f, py = 0.0f, pz = 0.0f, tu = 0.0f, tv = 0.0f, tw = 0.0f;
         float mass[FP_MAX_SPECIES];
         if (inb) {
-            flags = s.u(F_FLAGS)[i];
-            cx = s.i(F_CELL_X)[i]; cy = s.i(F_CELL_Y)[i];
-            px = s.f(F_POS_X)[i]; py = s.f(F_POS_Y)[i]; pz = s.f(F_POS_Z)[i];
-            tu = s.f(F_TURB_U)[i]; tv = s.f(F_TURB_V)[i]; tw = s.f(F_TURB_W)[i];
-            cbt = s.i(F_CBT)[i];
+            flags = sl.u(F_FLAGS)[g];
+            cx = sl.i(F_CELL_X)[g]; cy = sl.i(F_CELL_Y)[g];
+            px = sl.f(F_POS_X)[g]; py = sl.f(F_POS_Y)[g]; pz = sl.f(F_POS_Z)[g];
+            tu = sl.f(F_TURB_U)[g]; tv = sl.f(F_TURB_V)[g]; tw = sl.f(F_TURB_W)[g];
+            cbt = sl.i(F_CBT)[g];
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) mass[q] = s.f(F_MASS0 + q)[i];
+                if (q < a.n_species) mass[q] = sl.f(F_MASS0 + q)[g];
         }
 
         // ---- Philox blocks per slot (langevin_fused.wgsl:430-451) and the
         // Box-Muller normals: they depend on the slot index only, so they are
         // computed while the state loads are in flight.
-        const uint64_t cbase = i * a.blocks_per_particle;
+        const uint64_t cbase = at * a.blocks_per_particle;
         const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
         U4 b1 = U4{0u, 0u, 0u, 0u};
         if (two_blocks) b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
@@ -122,7 +140,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.dry_mode != FP_DRY_OFF) {
                 const float href = fmaxf(a.dry_href, 0.1f);
                 if (!(pz >= 2.0f * href)) {
-                    const float vd = a.dry_mode == FP_DRY_PER_SLOT ? a.dry_vd_slots[i] : a.dry_vd;
+                    const float vd = a.dry_mode == FP_DRY_PER_SLOT ? a.dry_vd_slots[at] : a.dry_vd;
                     const float ex = -fmaxf(vd, 0.0f) * fabsf(a.dt) / (2.0f * href);
                     const float sv = clampf(fpm_expf(ex), 0.0f, 1.0f);
 #pragma unroll
@@ -134,8 +152,8 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.wet_mode != FP_WET_OFF) {
                 float lam, frac;
                 if (a.wet_mode == FP_WET_PER_SLOT) {
-                    lam = a.wet_lambda_slots[i];
-                    frac = a.wet_fraction_slots[i];
+                    lam = a.wet_lambda_slots[at];
+                    frac = a.wet_fraction_slots[at];
                 } else if (a.wet_mode == FP_WET_PRECIP_FIELD) {
                     // below_cloud_scavenging_coefficient_gas (physics/wet_scavenging.rs:213-225)
                     const float pr = pos_or_zero(a.precip[pidx]);
@@ -176,6 +194,12 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
                 grid_q = quantize_mass(mass[a.grid.species], a.grid.mass_scale);
             }
         }
+        if (GATHER != kGatherNone && inb && !live) {
+            // a slot the step leaves unchanged still moves its stepped words
+#pragma unroll
+            for (uint32_t f = 0; f < F_COUNT; ++f)
+                if (stepped_field(f, a.n_species)) s.u(f)[i] = sl.u(f)[g];
+        }
         if (a.grid_fused) grid_reduce_wave(gt, a.grid, grid_valid, grid_c, grid_q);
     }
 }
@@ -193,7 +217,7 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 // The kernel is latency-bound at low occupancy; the register budget is
 // capped for 3 wavefronts per SIMD (<= 168 VGPRs), the most that fits
 // without spilling to scratch.
-template <uint32_t BLOCK, int NSUB>
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone>
 __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
 {
     __shared__ float lh[kMaxLevels];
@@ -202,7 +226,7 @@ __global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
     if (a.grid_fused) grid_lds_clear<BLOCK>(gt);
     load_tables<BLOCK>(a, lh, oh);
     const uint64_t first = (uint64_t)xcd_chunk(blockIdx.x, gridDim.x) * BLOCK * a.ppt;
-    step_chunk<BLOCK, NSUB>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
+    step_chunk<BLOCK, NSUB, GATHER>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
     if (a.grid_fused) grid_lds_flush<BLOCK>(gt, a.grid);
 }
 
@@ -289,6 +313,31 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st)
     case 1024: return launch_block<1024>(a, grid, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// A step fused with the record movement of a compaction: slot i of a.s gets
+// the stepped words (stepped_field) of src slot gather_map[i] advanced by one
+// step; the record's other words are not written (the caller keeps them equal
+// in both sets).  at_source_slot = false: the compaction came before the step
+// (the particle steps at slot i, exactly as after launch_compact), so the
+// compacting loop never copies records separately; true: the compaction after
+// the step is already known (a recomputation that replays its survivor map),
+// so only the survivors are stepped, each at its own slot.
+template <int GATHER>
+hipError_t launch_gathered(const StepArgs &a, hipStream_t st)
+{
+    if (a.n == 0) return hipSuccess;
+    if (!a.gather_map || a.src.base == a.s.base) return hipErrorInvalidValue;
+    const uint64_t per_block = 256ull * a.ppt;
+    const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
+    if (a.n_substeps == 4u) step_kernel<256, 4, GATHER><<<grid, 256, 0, st>>>(a);
+    else step_kernel<256, -1, GATHER><<<grid, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st)
+{
+    return at_source_slot ? launch_gathered<kGatherFromSlot>(a, st) : launch_gathered<kGatherToSlot>(a, st);
 }
 
 template <uint32_t BLOCK>

@@ -550,3 +550,47 @@ def test_emission_only_backward_matches_the_full_adjoint_through_losses(fp, case
     assert counts[-1] < counts[0]
     np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-5, atol=1e-7 * np.max(np.abs(out[0][0])))
     assert np.all(np.isfinite(out[0][1])) and np.all(np.isfinite(out[0][2]))
+
+
+def test_compacting_backward_matches_central_differences(fp, case):
+    """The backward recomputes each segment by stepping only the survivors
+    of every step, each at its own slot, through the saved maps.  With
+    deposition off and a loss threshold between the sources' particle masses,
+    whole sources die in the first step (the later survivors move down) and
+    the loss is exactly quadratic in the emissions, so the central difference
+    at relative step 1e-3 resolves the gradient; a recomputed state that
+    differed from the forward's would change it."""
+    c = case
+    masses = np.sort(np.unique((c.emission / c.counts.astype(np.float32)).astype(np.float32)))
+    assert len(masses) >= 4
+    thr = float(np.sqrt(masses[1] * masses[2]))
+    dead = (c.emission / c.counts.astype(np.float32)) < thr
+    first_dead = int(np.argmax(dead))
+    assert dead.sum() == 2 and np.any(~dead[first_dead:]), "later survivors must move"
+    cfg = cfg_of(fp, c, dry_mode=0, loss_mode=1, loss_min_mass=thr)
+    nodes = all_nodes(c)
+    with make_adj(fp, c, compact=True) as adj:
+        adj.set_receptors(nodes)
+        base, _ = adj.forward(cfg, c.emission, c.position, c.coeff)
+    hit = np.nonzero(base > 0)[0]
+    obs = base[hit] * np.random.default_rng(43).uniform(0.5, 1.5, len(hit))
+    say(f"compacting backward vs central differences: {len(hit)} receptors, sources lost {np.nonzero(dead)[0]}")
+    for emission_only in (True, False):
+        with make_adj(fp, c, compact=True, emission_only=emission_only) as adj:
+            adj.set_receptors(nodes[hit], obs)
+            adj.forward(cfg, c.emission, c.position, c.coeff)
+            counts = adj.active_counts()
+            gq, _, _ = adj.backward()
+            fd = np.zeros(len(COUNTS))
+            for s in range(len(COUNTS)):
+                h = np.float32(1e-3) * c.emission[s]
+                out = []
+                for sgn in (1.0, -1.0):
+                    q = c.emission.copy()
+                    q[s] = np.float32(q[s] + sgn * h)
+                    out.append(adj.forward(cfg, q, c.position, c.coeff)[1])
+                fd[s] = (out[0] - out[1]) / float(np.float32(c.emission[s] + h) - np.float32(c.emission[s] - h))
+        say(f"  emission_only={emission_only}: counts {list(counts)} adjoint {gq} fd {fd}")
+        assert counts[1] == sum(COUNTS) - int(np.sum(c.counts[dead])) == counts[-1]
+        assert np.all(gq[dead] == 0.0) and np.all(np.abs(fd[dead]) <= 1e-6 * np.max(np.abs(fd)))
+        np.testing.assert_allclose(gq, fd, rtol=1e-3, atol=1e-4 * np.max(np.abs(fd)))
