@@ -302,8 +302,9 @@ def bench_ensemble(args) -> int:
     """96 members (lognormal sizes, median 200k, about 32M particles) released
     over the first 6 h, each with its own 10 % wind perturbation of the global
     360 x 181 x 60 met, its own Philox key and its own 360 x 181 x 5 grid.
-    After the releases (untimed) and an untimed launch-shape sweep, a 36-step
-    window (6 h, 600 s steps) is timed four ways: a loop over per-member
+    After the releases (untimed) and the ensemble autotuner (untimed: block
+    size, particles/thread and re-sort interval), a 36-step window (6 h, 600 s
+    steps) is timed four ways: a loop over per-member
     contexts issuing every member's fp_step each step with no host wait (each
     context on its own stream), the plain member-after-member loop with a
     stream synchronisation after each step, the ensemble with one direct
@@ -387,23 +388,27 @@ def bench_ensemble(args) -> int:
     print(f"# ensemble: {M} members, {total} particles, releases done in "
           f"{time.perf_counter() - t_gen:.1f} s; members equal single runs", file=sys.stderr, flush=True)
 
-    # launch-shape sweep on the ensemble's graph path (untimed)
-    best = None
-    for block in (128, 256, 512):
-        for ppt in (1, 2, 4):
-            ens.set_launch(block, ppt)
-            ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
-            ens_ctr = ens.step(2, use_graph=True)
-            ens_ctr = ens.step(6, use_graph=True)
-            ms = ens.last_step_ms()
-            if best is None or ms < best[0]:
-                best = (ms, block, ppt)
-    _, block, ppt = best
-    ens.set_launch(block, ppt)
+    # launch shape and re-sort interval from the ensemble autotuner on the
+    # state after the releases (untimed; the tuner restores the state)
+    ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
+    tuned = ens.autotune(sort_intervals=(0, 12, 36), warmup_steps=2, measured_steps=6, sort_trial_steps=36)
+    block, ppt, sort_every = tuned["block_size"], tuned["particles_per_thread"], tuned["sort_interval"]
     for c in loop:
         c.set_launch(block, ppt)
-    print(f"# ensemble launch sweep: block {block}, {ppt} particles/thread, {best[0]:.3f} ms/step",
-          file=sys.stderr, flush=True)
+    print(f"# ensemble autotune: {tuned}", file=sys.stderr, flush=True)
+
+    def ens_run(n_steps, use_graph):
+        """n_steps ensemble steps with the tuned re-sort interval."""
+        if not sort_every:
+            return ens.step(n_steps, use_graph=use_graph)
+        done, ctr = 0, None
+        while done < n_steps:
+            if done % sort_every == 0:
+                ens.sort_morton(10)
+            k = min(sort_every - done % sort_every, n_steps - done)
+            ctr = ens.step(k, use_graph=use_graph)
+            done += k
+        return ctr
 
     # (a) loop over members: one fp_step per member per step, parameter
     # blocks prepared up front so the host loop is ctypes calls only
@@ -442,15 +447,15 @@ def bench_ensemble(args) -> int:
     ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
     ens.sync()
     t0 = time.perf_counter()
-    ens_ctr = ens.step(args.steps, use_graph=False)
+    ens_ctr = ens_run(args.steps, use_graph=False)
     t_direct = time.perf_counter() - t0
     ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
     ens_ctr = ens.step(1, use_graph=True)  # capture
     ens.sync()
     t0 = time.perf_counter()
-    ens_ctr = ens.step(args.steps, use_graph=True)
+    ens_ctr = ens_run(args.steps, use_graph=True)
     t_graph = time.perf_counter() - t0
-    step_ms = ens.last_step_ms()
+    step_ms = t_graph * 1e3 / args.steps if sort_every else ens.last_step_ms()
 
     ps = total * args.steps
     achieved = total * BYTES_PER_PARTICLE_STEP / (step_ms / 1e3) / 1e9
@@ -475,6 +480,9 @@ def bench_ensemble(args) -> int:
             "particles": total,
             "block": block,
             "particles_per_thread": ppt,
+            "sort_interval": sort_every,
+            "autotune_step_ms": tuned["step_ms"],
+            "autotune_step_ms_with_sort": tuned["step_ms_with_sort"],
             "loop_streams_particle_steps_per_s": ps / t_loop,
             "loop_serial_particle_steps_per_s": ps / t_serial,
             "direct_particle_steps_per_s": ps / t_direct,

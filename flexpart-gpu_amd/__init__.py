@@ -264,6 +264,8 @@ ABI = {
     "fp_ens_step": (C.c_int, [_vp, C.c_uint32, C.c_int, _u32]),
     "fp_ens_set_launch": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "fp_ens_last_step_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "fp_ens_sort_morton": (C.c_int, [_vp, C.POINTER(SortParams)]),
+    "fp_ens_autotune": (C.c_int, [_vp, C.POINTER(AutotuneParams), C.POINTER(AutotuneResult)]),
 }
 
 _LIB: C.CDLL | None = None
@@ -870,6 +872,31 @@ class GpuEnsemble:
 
     def set_launch(self, block_size: int = 256, particles_per_thread: int = 1):
         self._chk(self._lib.fp_ens_set_launch(self._h, block_size, particles_per_thread))
+
+    def sort_morton(self, bits_per_axis: int = 10, bounds=None):
+        """Morton re-sort of every member's dispatch range (fp_ens_sort_morton)."""
+        sp = SortParams()
+        sp.bits_per_axis = bits_per_axis
+        if bounds is not None:
+            sp.explicit_bounds = 1
+            for i, v in enumerate(bounds):
+                sp.bounds[i] = float(v)
+        self._chk(self._lib.fp_ens_sort_morton(self._h, C.byref(sp)))
+
+    def autotune(self, sort_intervals=(0, 6, 12, 36), warmup_steps: int = 2, measured_steps: int = 7,
+                 sort_trial_steps: int = 36, sort_bits: int = 10) -> dict:
+        """Pick block size, particles/thread and re-sort interval of the graph
+        path on the current state (fp_ens_autotune); the state is kept."""
+        p = AutotuneParams()
+        p.warmup_steps, p.measured_steps = warmup_steps, measured_steps
+        p.sort_bits, p.sort_trial_steps = sort_bits, sort_trial_steps
+        p.n_sort_intervals = len(sort_intervals)
+        for i, s in enumerate(sort_intervals):
+            p.sort_intervals[i] = s
+        r = AutotuneResult()
+        self._chk(self._lib.fp_ens_autotune(self._h, C.byref(p), C.byref(r)))
+        return {"block_size": r.block_size, "particles_per_thread": r.particles_per_thread,
+                "sort_interval": r.sort_interval, "step_ms": r.step_ms, "step_ms_with_sort": r.step_ms_with_sort}
 
     def last_step_ms(self) -> float:
         ms = C.c_double()

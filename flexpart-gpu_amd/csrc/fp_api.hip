@@ -153,9 +153,8 @@ void destroy(fp_ctx *ctx)
     fr(ctx->grid_counts); fr(ctx->grid_mass);
     fr(ctx->aos_stage); fr(ctx->slot_stage);
     fr(ctx->block_counts); fr(ctx->survivor_map); fr(ctx->active_count_dev);
-    fr(ctx->bounds_partial); fr(ctx->bounds_dev);
-    fr(ctx->sort_keys); fr(ctx->sort_keys_alt); fr(ctx->sort_vals); fr(ctx->sort_vals_alt);
-    fr(ctx->sort_temp); fr(ctx->release_jobs);
+    fp::sort_work_free(ctx->sortw);
+    fr(ctx->release_jobs);
     if (ctx->active_count_host) (void)hipHostFree(ctx->active_count_host);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     for (auto &ev : ctx->free_events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
@@ -304,12 +303,7 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
     CREATE_HIP(dalloc(&ctx->survivor_map, d.capacity));
     CREATE_HIP(dalloc(&ctx->active_count_dev, 1));
     CREATE_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->active_count_host), sizeof(uint32_t)));
-    CREATE_HIP(dalloc(&ctx->bounds_partial, 2048 * 7));
-    CREATE_HIP(dalloc(&ctx->bounds_dev, 8));
-    CREATE_HIP(dalloc(reinterpret_cast<uint64_t **>(&ctx->sort_keys), d.capacity));
-    CREATE_HIP(dalloc(reinterpret_cast<uint64_t **>(&ctx->sort_keys_alt), d.capacity));
-    CREATE_HIP(dalloc(&ctx->sort_vals, d.capacity));
-    CREATE_HIP(dalloc(&ctx->sort_vals_alt, d.capacity));
+    CREATE_HIP(fp::sort_work_alloc(ctx->sortw, d.capacity));
     CREATE_HIP(hipStreamSynchronize(ctx->stream));
 #undef CREATE_HIP
     ctx->segs.reset(d.capacity, 0);
@@ -946,13 +940,14 @@ int fp_sort_morton(fp_ctx *ctx, const fp_sort_params *p, uint32_t *new_to_old)
     if (int rc = ensure_stage(ctx, n)) return rc;
     fp_ctx::Ev ev{};
     prof_begin(ctx, ev, FP_KERNEL_SORT);
-    FP_HIP(ctx, fp::sort_morton(ctx, *p, n));
+    FP_HIP(ctx, fp::sort_morton_range(ctx->sortw, *p, ctx->soa[ctx->cur], ctx->soa[1 - ctx->cur], ctx->aos_stage, n,
+                                      ctx->stream));
     prof_end(ctx, ev);
     const int other = 1 - ctx->cur;
     ctx->segs.assign(0, n, other);
     ctx->cur = other;
     if (new_to_old) {
-        FP_HIP(ctx, hipMemcpyAsync(new_to_old, ctx->sort_vals_alt, (uint64_t)n * 4,
+        FP_HIP(ctx, hipMemcpyAsync(new_to_old, ctx->sortw.vals_alt, (uint64_t)n * 4,
                                    hipMemcpyDeviceToHost, ctx->stream));
         FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }

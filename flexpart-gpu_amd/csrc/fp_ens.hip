@@ -53,6 +53,7 @@ struct fp_ens {
     uint64_t steps_done = 0;
 
     uint32_t block = 256, ppt = 1;
+    fp::SortWork sortw;  // allocated by the first fp_ens_sort_morton
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -113,6 +114,7 @@ void destroy(fp_ens *e)
     fr(e->soa.base); fr(e->met); fr(e->met_stage); fr(e->level_heights); fr(e->pbl); fr(e->precip); fr(e->pbl_stage);
     fr(e->grid_counts); fr(e->grid_mass); fr(e->aos_stage); fr(e->d_members); fr(e->d_blocks);
     fr(e->d_clock);
+    fp::sort_work_free(e->sortw);
     if (e->ev_a) (void)hipEventDestroy(e->ev_a);
     if (e->ev_b) (void)hipEventDestroy(e->ev_b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -149,6 +151,37 @@ int met_member_common(fp_ens *e, uint32_t m, const float *const f[6], hipMemcpyK
                                  (uint32_t)np, e->stream));
     E_HIP(e, hipStreamSynchronize(e->stream));
     e->met_ready[m] = 1;
+    return FP_OK;
+}
+
+// Launch tables for the current block size and particles per thread from the
+// configured members (the clock is left as it is).
+int rebuild_tables(fp_ens *e)
+{
+    drop_graph(e);
+    std::vector<fp::EnsBlock> blocks;
+    const uint64_t per_block = (uint64_t)e->block * e->ppt;
+    for (uint32_t m = 0; m < e->n_members; ++m) {
+        e->members[m].a.ppt = e->ppt;
+        for (uint64_t f = 0; f < e->members[m].a.n; f += per_block) blocks.push_back(fp::EnsBlock{m, (uint32_t)f});
+    }
+    e->nblocks = (uint32_t)blocks.size();
+    if (blocks.size() > e->d_blocks_cap) {
+        if (e->d_blocks) {
+            E_HIP(e, hipStreamSynchronize(e->stream));
+            (void)hipFree(e->d_blocks);
+        }
+        e->d_blocks = nullptr;
+        e->d_blocks_cap = 0;
+        E_HIP(e, dmalloc(&e->d_blocks, blocks.size()));
+        e->d_blocks_cap = blocks.size();
+    }
+    E_HIP(e, hipMemcpyAsync(e->d_members, e->members.data(), e->members.size() * sizeof(fp::EnsMember),
+                            hipMemcpyHostToDevice, e->stream));
+    if (!blocks.empty())
+        E_HIP(e, hipMemcpyAsync(e->d_blocks, blocks.data(), blocks.size() * sizeof(fp::EnsBlock),
+                                hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, hipStreamSynchronize(e->stream));
     return FP_OK;
 }
 
@@ -355,8 +388,9 @@ int fp_ens_set_launch(fp_ens *e, uint32_t block, uint32_t ppt)
     E_CHECK(e, ppt >= 1 && ppt <= 16, "particles_per_thread must be 1..16");
     e->block = block;
     e->ppt = ppt;
-    e->configured = false;
     drop_graph(e);
+    // a configured ensemble keeps its configuration and clock
+    if (e->configured) return rebuild_tables(e);
     return FP_OK;
 }
 
@@ -377,8 +411,6 @@ int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, c
     E_CHECK(e, bracket_s > 0.0 && std::isfinite(t0_in_bracket), "invalid bracket time");
     drop_graph(e);
     e->members.assign(e->n_members, fp::EnsMember{});
-    std::vector<fp::EnsBlock> blocks;
-    const uint64_t per_block = (uint64_t)e->block * e->ppt;
     const uint64_t np = e->npts();
     for (uint32_t m = 0; m < e->n_members; ++m) {
         const uint64_t n = dispatch ? dispatch[m] : e->cap[m];
@@ -407,26 +439,13 @@ int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, c
         a.ppt = e->ppt;
         for (int k = 0; k < 4; ++k) e->members[m].base_ctr[k] = counters ? counters[4 * m + k] : p->counter[k];
         e->members[m].ctr_per_step = fp::counter_per_step(n, p->n_substeps);
-        for (uint64_t f = 0; f < n; f += per_block) blocks.push_back(fp::EnsBlock{m, (uint32_t)f});
     }
-    e->nblocks = (uint32_t)blocks.size();
-    if (blocks.size() > e->d_blocks_cap) {
-        if (e->d_blocks) (void)hipFree(e->d_blocks);
-        e->d_blocks = nullptr;
-        e->d_blocks_cap = 0;
-        E_HIP(e, dmalloc(&e->d_blocks, blocks.size()));
-        e->d_blocks_cap = blocks.size();
-    }
+    if (int rc = rebuild_tables(e)) return rc;
     fp::EnsClock clk{};
     clk.next_step = 0;
     clk.t0_in_bracket = t0_in_bracket;
     clk.dt_seconds = p->direction < 0 ? -(double)p->dt_seconds : (double)p->dt_seconds;
     clk.bracket_seconds = bracket_s;
-    E_HIP(e, hipMemcpyAsync(e->d_members, e->members.data(), e->members.size() * sizeof(fp::EnsMember),
-                            hipMemcpyHostToDevice, e->stream));
-    if (!blocks.empty())
-        E_HIP(e, hipMemcpyAsync(e->d_blocks, blocks.data(), blocks.size() * sizeof(fp::EnsBlock),
-                                hipMemcpyHostToDevice, e->stream));
     E_HIP(e, hipMemcpyAsync(e->d_clock, &clk, sizeof clk, hipMemcpyHostToDevice, e->stream));
     E_HIP(e, hipStreamSynchronize(e->stream));
     e->sub4 = p->n_substeps == 4;
@@ -474,6 +493,147 @@ int fp_ens_last_step_ms(fp_ens *e, double *ms)
 {
     E_CHECK(e, e && ms, "null argument");
     *ms = e->last_ms;
+    return FP_OK;
+}
+
+// Morton re-sort of every member's dispatch range (the member's capacity
+// before the first fp_ens_configure): the order fp_sort_morton gives for the
+// member alone.  Members are sorted in place through the AoS stage.
+int fp_ens_sort_morton(fp_ens *e, const fp_sort_params *p)
+{
+    E_CHECK(e, e && p, "null argument");
+    E_CHECK(e, p->bits_per_axis >= 1 && p->bits_per_axis <= 21, "bits_per_axis must be 1..21");
+    if (!e->sortw.cap) {
+        const hipError_t h = fp::sort_work_alloc(e->sortw, e->max_cap);
+        if (h != hipSuccess) {
+            fp::sort_work_free(e->sortw);
+            E_HIP(e, h);
+        }
+    }
+    for (uint32_t m = 0; m < e->n_members; ++m) {
+        const uint64_t n = e->members.empty() ? e->cap[m] : e->members[m].a.n;
+        if (n == 0) continue;
+        fp::Soa s = e->soa;
+        s.base += e->off[m];
+        E_HIP(e, fp::sort_morton_range(e->sortw, *p, s, s, e->aos_stage, (uint32_t)n, e->stream));
+    }
+    E_HIP(e, hipStreamSynchronize(e->stream));
+    return FP_OK;
+}
+
+// Launch shape and re-sort interval of the ensemble's graph path: block size x
+// particles per thread timed on the current state (warm-up, then the median of
+// measured steps), then the re-sort intervals with the winning shape, timed
+// with the sorts amortised; ties keep the earlier candidate (the reference's
+// workgroup autotuner, gpu/workgroup.rs:342-445).  The particle state, the
+// grids and the step clock are restored afterwards, so the run continues
+// exactly as without the call.  params->cache_path is not used.
+int fp_ens_autotune(fp_ens *e, const fp_autotune_params *params, fp_autotune_result *out)
+{
+    E_CHECK(e, e && params && out, "null argument");
+    E_STATE(e, e->configured, "fp_ens_autotune before fp_ens_configure");
+    E_CHECK(e, params->measured_steps >= 1 && params->n_sort_intervals <= 8,
+            "measured_steps must be >= 1 and n_sort_intervals <= 8");
+    std::memset(out, 0, sizeof *out);
+    const uint64_t words = e->total * fp::F_COUNT;
+    const uint64_t gwords = e->grid_counts ? e->ncell() * e->n_members : 0;
+    uint32_t *snap = nullptr;
+    E_HIP(e, dmalloc(&snap, words + 2 * gwords));
+    fp::EnsClock clk{};
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t h = hipMemcpyAsync(save ? snap : e->soa.base, save ? e->soa.base : snap, words * 4,
+                                      hipMemcpyDeviceToDevice, e->stream);
+        if (h == hipSuccess && gwords)
+            h = hipMemcpyAsync(save ? snap + words : e->grid_counts, save ? e->grid_counts : snap + words,
+                               gwords * 4, hipMemcpyDeviceToDevice, e->stream);
+        if (h == hipSuccess && gwords)
+            h = hipMemcpyAsync(save ? snap + words + gwords : e->grid_mass,
+                               save ? e->grid_mass : snap + words + gwords, gwords * 4, hipMemcpyDeviceToDevice,
+                               e->stream);
+        if (h == hipSuccess)
+            h = save ? hipMemcpyAsync(&clk, e->d_clock, sizeof clk, hipMemcpyDeviceToHost, e->stream)
+                     : hipMemcpyAsync(e->d_clock, &clk, sizeof clk, hipMemcpyHostToDevice, e->stream);
+        if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+        return h;
+    };
+    const uint32_t block0 = e->block, ppt0 = e->ppt;
+    const uint64_t steps0 = e->steps_done;
+    int rc = FP_OK;
+    hipEvent_t ta = nullptr, tb = nullptr;  // fp_ens_step records ev_a/ev_b itself
+    hipError_t h = hipEventCreate(&ta);
+    if (h == hipSuccess) h = hipEventCreate(&tb);
+    if (h == hipSuccess) h = copy_all(true);
+    auto timed = [&](uint32_t steps, uint32_t sort_every, std::vector<double> *per_step, double *mean) -> int {
+        fp_sort_params sp{};
+        sp.bits_per_axis = params->sort_bits ? params->sort_bits : 10;
+        double total = 0.0;
+        for (uint32_t k = 0; k < steps; ++k) {
+            E_HIP(e, hipEventRecord(ta, e->stream));
+            if (sort_every && k % sort_every == 0)
+                if (int r = fp_ens_sort_morton(e, &sp)) return r;
+            if (int r = fp_ens_step(e, 1, 1, nullptr)) return r;
+            E_HIP(e, hipEventRecord(tb, e->stream));
+            E_HIP(e, hipEventSynchronize(tb));
+            float ms = 0.0f;
+            E_HIP(e, hipEventElapsedTime(&ms, ta, tb));
+            if (per_step) per_step->push_back(ms);
+            total += ms;
+        }
+        if (mean) *mean = steps ? total / steps : 0.0;
+        return FP_OK;
+    };
+    static const uint32_t blocks[] = {128, 256, 512};
+    static const uint32_t ppts[] = {1, 2, 4};
+    double best = 1e300;
+    uint32_t best_b = block0, best_p = ppt0, best_s = 0;
+    if (h != hipSuccess) goto done;
+    for (uint32_t b : blocks)
+        for (uint32_t pp : ppts) {
+            if ((rc = fp_ens_set_launch(e, b, pp))) goto done;
+            if ((h = copy_all(false)) != hipSuccess) goto done;
+            if ((rc = timed(params->warmup_steps, 0, nullptr, nullptr))) goto done;
+            std::vector<double> t;
+            if ((rc = timed(params->measured_steps, 0, &t, nullptr))) goto done;
+            std::sort(t.begin(), t.end());
+            if (t[t.size() / 2] < best) {
+                best = t[t.size() / 2];
+                best_b = b;
+                best_p = pp;
+            }
+        }
+    out->step_ms = best;
+    out->step_ms_with_sort = best;
+    if ((rc = fp_ens_set_launch(e, best_b, best_p))) goto done;
+    for (uint32_t c = 0; c < params->n_sort_intervals; ++c) {
+        const uint32_t every = params->sort_intervals[c];
+        const uint32_t steps = std::max<uint32_t>(params->sort_trial_steps, 1u);
+        if ((h = copy_all(false)) != hipSuccess) goto done;
+        double mean = 0.0;
+        if ((rc = timed(steps, every, nullptr, &mean))) goto done;
+        if (c == 0 || mean < out->step_ms_with_sort) {
+            out->step_ms_with_sort = mean;
+            best_s = every;
+        }
+    }
+done:
+    {
+        // the state, grids and clock as before the call, with the chosen shape
+        // (or the original one after an error)
+        const int rc2 = fp_ens_set_launch(e, rc == FP_OK && h == hipSuccess ? best_b : block0,
+                                          rc == FP_OK && h == hipSuccess ? best_p : ppt0);
+        const hipError_t h2 = copy_all(false);
+        e->steps_done = steps0;
+        (void)hipFree(snap);
+        if (ta) (void)hipEventDestroy(ta);
+        if (tb) (void)hipEventDestroy(tb);
+        E_HIP(e, h);
+        if (rc) return rc;
+        if (rc2) return rc2;
+        E_HIP(e, h2);
+    }
+    out->block_size = best_b;
+    out->particles_per_thread = best_p;
+    out->sort_interval = best_s;
     return FP_OK;
 }
 

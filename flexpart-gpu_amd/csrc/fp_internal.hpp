@@ -141,6 +141,22 @@ struct Segments {
 
 }  // namespace fp
 
+namespace fp {
+// Device work buffers of a Morton sort of up to `cap` slots (sort_morton_range).
+struct SortWork {
+    double *bounds_partial = nullptr;  // kSortPartials * 7
+    double *bounds_dev = nullptr;      // 8
+    void *keys = nullptr, *keys_alt = nullptr;        // cap u64 keys each
+    uint32_t *vals = nullptr, *vals_alt = nullptr;    // cap slots each; vals_alt = new_to_old after a sort
+    void *temp = nullptr;                             // rocPRIM temporary storage, grown on demand
+    size_t temp_bytes = 0;
+    uint64_t cap = 0;
+};
+constexpr uint32_t kSortPartials = 2048;
+hipError_t sort_work_alloc(SortWork &w, uint64_t cap);
+void sort_work_free(SortWork &w);
+}  // namespace fp
+
 struct fp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -195,13 +211,7 @@ struct fp_ctx {
     uint32_t *active_count_host = nullptr;  // pinned
 
     // sort
-    double *bounds_partial = nullptr;
-    uint64_t bounds_partial_cap = 0;
-    double *bounds_dev = nullptr;
-    void *sort_keys = nullptr, *sort_keys_alt = nullptr;
-    uint32_t *sort_vals = nullptr, *sort_vals_alt = nullptr;
-    void *sort_temp = nullptr;
-    size_t sort_temp_bytes = 0;
+    fp::SortWork sortw;
 
     // asynchronous bracket prefetch
     hipStream_t copy_stream = nullptr;
@@ -291,7 +301,10 @@ int write_netcdf(const char *path, const fp_netcdf_snapshot &s, int version, std
 // (nx*ny doubles, may be null) accumulate the adjoint of the horizontal wind.
 hipError_t launch_adj_step(const StepArgs &a, float *lam, uint64_t stride, double *au, double *av, hipStream_t st);
 // Morton sort: keys + stable radix sort + gather (fp_sort.hip).
-hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n);
+// Morton sort of slots [0, n) of `in` into `out` (which may be `in`: the
+// records are gathered from an AoS copy in aos_stage, >= n records).
+hipError_t sort_morton_range(SortWork &w, const fp_sort_params &p, const Soa &in, const Soa &out,
+                             fp_particle *aos_stage, uint32_t n, hipStream_t st);
 
 // Shared by the single-run and ensemble entry points (fp_api.hip).
 // Scalar step arguments (timestep, scales, Philox key and counter, sub-steps,

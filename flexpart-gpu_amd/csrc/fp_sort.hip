@@ -10,6 +10,8 @@
 // (active by key, ties and inactive slots in original order).
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <algorithm>
+
 #include "fp_internal.hpp"
 
 namespace fp {
@@ -158,67 +160,86 @@ __global__ void gather_records_kernel(const uint4 *aos, Soa out, uint32_t n, con
 }
 
 template <typename K>
-hipError_t sort_pairs(fp_ctx *ctx, uint32_t n, uint32_t end_bit)
+hipError_t sort_pairs(SortWork &w, uint32_t n, uint32_t end_bit, hipStream_t st)
 {
-    K *k_in = static_cast<K *>(ctx->sort_keys);
-    K *k_out = static_cast<K *>(ctx->sort_keys_alt);
+    K *k_in = static_cast<K *>(w.keys);
+    K *k_out = static_cast<K *>(w.keys_alt);
     size_t need = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, need, k_in, k_out, ctx->sort_vals,
-                                             ctx->sort_vals_alt, (size_t)n, 0u, end_bit, ctx->stream);
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, need, k_in, k_out, w.vals, w.vals_alt, (size_t)n, 0u,
+                                             end_bit, st);
     if (e != hipSuccess) return e;
-    if (need > ctx->sort_temp_bytes) {
-        if (ctx->sort_temp) (void)hipFree(ctx->sort_temp);
-        ctx->sort_temp = nullptr;
-        e = hipMalloc(&ctx->sort_temp, need);
+    if (need > w.temp_bytes) {
+        if (w.temp) {
+            e = hipStreamSynchronize(st);  // a previous sort may still use it
+            if (e != hipSuccess) return e;
+            (void)hipFree(w.temp);
+        }
+        w.temp = nullptr;
+        w.temp_bytes = 0;
+        e = hipMalloc(&w.temp, need);
         if (e != hipSuccess) return e;
-        ctx->sort_temp_bytes = need;
+        w.temp_bytes = need;
     }
-    return rocprim::radix_sort_pairs(ctx->sort_temp, need, k_in, k_out, ctx->sort_vals,
-                                     ctx->sort_vals_alt, (size_t)n, 0u, end_bit, ctx->stream);
+    return rocprim::radix_sort_pairs(w.temp, need, k_in, k_out, w.vals, w.vals_alt, (size_t)n, 0u, end_bit, st);
 }
 
 }  // namespace
 
-// Sorts slots [0, n) of the current set into the other set; the caller swaps
-// sets.  On return ctx->sort_vals_alt holds new_to_old.
-hipError_t sort_morton(fp_ctx *ctx, const fp_sort_params &p, uint32_t n)
+hipError_t sort_work_alloc(SortWork &w, uint64_t cap)
 {
-    hipStream_t st = ctx->stream;
-    const Soa in = ctx->soa[ctx->cur];
-    const Soa out = ctx->soa[1 - ctx->cur];
+    cap = std::max<uint64_t>(cap, 1);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&w.bounds_partial), kSortPartials * 7 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&w.bounds_dev), 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&w.keys, cap * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&w.keys_alt, cap * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&w.vals), cap * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&w.vals_alt), cap * sizeof(uint32_t));
+    if (e == hipSuccess) w.cap = cap;
+    return e;
+}
+
+void sort_work_free(SortWork &w)
+{
+    for (void *q : {(void *)w.bounds_partial, (void *)w.bounds_dev, w.keys, w.keys_alt, (void *)w.vals,
+                    (void *)w.vals_alt, w.temp})
+        if (q) (void)hipFree(q);
+    w = SortWork{};
+}
+
+hipError_t sort_morton_range(SortWork &w, const fp_sort_params &p, const Soa &in, const Soa &out,
+                             fp_particle *aos_stage, uint32_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    if (n > w.cap) return hipErrorInvalidValue;
     hipError_t e;
     if (p.explicit_bounds) {
         double b[7];
         for (int k = 0; k < 6; ++k) b[k] = p.bounds[k];
         b[6] = 1.0;
-        e = hipMemcpyAsync(ctx->bounds_dev, b, sizeof b, hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(w.bounds_dev, b, sizeof b, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);  // b lives on this stack frame
         if (e != hipSuccess) return e;
     } else {
-        const uint32_t nparts = (uint32_t)std::min<uint64_t>((n + kRB - 1) / kRB, 2048);
-        bounds_partial_kernel<<<nparts, kRB, 0, st>>>(in, n, ctx->bounds_partial);
-        bounds_final_kernel<<<1, kRB, 0, st>>>(ctx->bounds_partial, nparts, ctx->bounds_dev);
+        const uint32_t nparts = (uint32_t)std::min<uint64_t>((n + kRB - 1) / kRB, kSortPartials);
+        bounds_partial_kernel<<<nparts, kRB, 0, st>>>(in, n, w.bounds_partial);
+        bounds_final_kernel<<<1, kRB, 0, st>>>(w.bounds_partial, nparts, w.bounds_dev);
     }
     const uint32_t grid = (n + kRB - 1) / kRB;
     const uint32_t end_bit = 3u * p.bits_per_axis + 1u;
     if (end_bit <= 32u) {
-        keys_kernel<uint32_t><<<grid, kRB, 0, st>>>(in, n, ctx->bounds_dev, p.bits_per_axis,
-                                                    static_cast<uint32_t *>(ctx->sort_keys),
-                                                    ctx->sort_vals);
-        e = sort_pairs<uint32_t>(ctx, n, end_bit);
+        keys_kernel<uint32_t><<<grid, kRB, 0, st>>>(in, n, w.bounds_dev, p.bits_per_axis,
+                                                    static_cast<uint32_t *>(w.keys), w.vals);
+        e = sort_pairs<uint32_t>(w, n, end_bit, st);
     } else {
-        keys_kernel<uint64_t><<<grid, kRB, 0, st>>>(in, n, ctx->bounds_dev, p.bits_per_axis,
-                                                    static_cast<uint64_t *>(ctx->sort_keys),
-                                                    ctx->sort_vals);
-        e = sort_pairs<uint64_t>(ctx, n, end_bit);
+        keys_kernel<uint64_t><<<grid, kRB, 0, st>>>(in, n, w.bounds_dev, p.bits_per_axis,
+                                                    static_cast<uint64_t *>(w.keys), w.vals);
+        e = sort_pairs<uint64_t>(w, n, end_bit, st);
     }
     if (e != hipSuccess) return e;
-    // ctx->aos_stage holds at least n records (ensured by the caller)
-    e = launch_soa_to_aos(in, ctx->aos_stage, n, st);
+    e = launch_soa_to_aos(in, aos_stage, n, st);
     if (e != hipSuccess) return e;
-    gather_records_kernel<<<grid, kRB, 0, st>>>(reinterpret_cast<const uint4 *>(ctx->aos_stage), out, n,
-                                                ctx->sort_vals_alt);
+    gather_records_kernel<<<grid, kRB, 0, st>>>(reinterpret_cast<const uint4 *>(aos_stage), out, n, w.vals_alt);
     return hipGetLastError();
 }
 

@@ -533,7 +533,22 @@ int fp_ens_configure(fp_ens *ens, const fp_step_params *base, const uint32_t *ke
  * (hipGraph) and replays it.  counters_out (4 per member, may be NULL)
  * receives each member's counter after the last step. */
 int fp_ens_step(fp_ens *ens, uint32_t n_steps, int use_graph, uint32_t *counters_out);
+/* Launch shape of the step; a configured ensemble keeps its configuration
+ * and step clock (the launch tables are rebuilt, a captured graph dropped). */
 int fp_ens_set_launch(fp_ens *ens, uint32_t block_size, uint32_t particles_per_thread);
+/* Morton re-sort of every member's dispatch range: each member ends in the
+ * order fp_sort_morton gives for it alone (replaces ParticleStore::
+ * sort_spatially, particles/mod.rs:649-699, per member).  The launch tables
+ * and a captured graph stay valid. */
+int fp_ens_sort_morton(fp_ens *ens, const fp_sort_params *params);
+/* Ensemble autotuner (the reference's workgroup autotuner, gpu/workgroup.rs:
+ * 342-445, over the ensemble's graph path): block size in {128, 256, 512} x
+ * particles per thread in {1, 2, 4} by the median of measured_steps steps
+ * after warmup_steps, then params->sort_intervals with the winning shape,
+ * each over sort_trial_steps steps with the sorts amortised.  Requires a
+ * configured ensemble; the particle state, grids and step clock are restored,
+ * and the winning launch shape is set.  cache_path is not used. */
+int fp_ens_autotune(fp_ens *ens, const fp_autotune_params *params, fp_autotune_result *out);
 /* mean milliseconds per fp_ens step launch over the last fp_ens_step call
  * (HIP events on the ensemble stream). Blocking. */
 int fp_ens_last_step_ms(fp_ens *ens, double *ms);

@@ -231,3 +231,75 @@ def test_ensemble_errors_are_reported(fp, small):
         ens.step(2)
         with pytest.raises(fp.FpError):
             ens.set_launch(100, 1)
+
+
+def configured(fp, small, ens, launch=None):
+    from test_gpu_parity import base_cfg
+    if launch:
+        ens.set_launch(*launch)
+    ens.configure(base_cfg(fp, small, **CASES["production"]), keys=[member_key(small, m) for m in range(len(CAPS))],
+                  counters=[member_counter(m) for m in range(len(CAPS))], dispatch=DISPATCH,
+                  time_in_bracket_s=T0, bracket_s=BRACKET)
+
+
+def test_member_sort_equals_the_single_context_sort(fp, small):
+    """fp_ens_sort_morton leaves every member in the order fp_sort_morton gives
+    for that member alone (its dispatch range; the slots beyond it untouched),
+    and the steps after it equal the single context's steps after its sort."""
+    say("ensemble Morton sort vs fp_sort_morton per member")
+    from test_gpu_parity import base_cfg
+    with build_ensemble(fp, small) as ens:
+        configured(fp, small, ens)
+        ctr3 = ens.step(3, use_graph=True)
+        before = [ens.download_member(m) for m in range(len(CAPS))]
+        ens.sort_morton(bits_per_axis=10)
+        sorted_ = [ens.download_member(m) for m in range(len(CAPS))]
+        ens.step(2, use_graph=True)   # the graph captured before the sort stays valid
+        after = [ens.download_member(m) for m in range(len(CAPS))]
+    for m in range(len(CAPS)):
+        lv = member_levels(small, m)
+        with fp.GpuTimestep(CAPS[m], small.met.shape, n_species=4) as ctx:
+            ctx.upload_met_bracket(*lv[0], *lv[1], small.met.level_heights)
+            ctx.upload_pbl(small.pbl.ustar, small.pbl.wstar, small.pbl.hmix, small.pbl.oli, small.pbl.precip)
+            ctx.upload_store(before[m])
+            ctx.set_dispatch_count(DISPATCH[m])
+            ctx.sort_morton(10, want_map=False)
+            assert_particles_equal(sorted_[m], ctx.download_particles(n=CAPS[m]), f"member {m} sorted")
+            c = base_cfg(fp, small, **CASES["production"])
+            c.grid_fused = False
+            c.key = member_key(small, m)
+            ctr = tuple(int(v) for v in ctr3[m])
+            for k in range(3, 5):
+                c.alpha = fp.interpolation_alpha(0, int(BRACKET), int(T0 + k * small.dt_seconds))
+                c.counter = ctr
+                ctr = ctx.step(c)
+            assert_particles_equal(after[m], ctx.download_particles(n=CAPS[m]), f"member {m} stepped after sort")
+
+
+def test_autotune_keeps_the_run_unchanged(fp, small):
+    """fp_ens_autotune times launch shapes and re-sort intervals on copies of
+    the state: afterwards the ensemble continues exactly like one that was
+    not tuned (results do not depend on the launch shape)."""
+    say("ensemble autotune: state, grids and clock restored")
+    outs = []
+    for tune in (False, True):
+        with build_ensemble(fp, small) as ens:
+            configured(fp, small, ens, launch=(256, 1))
+            ens.step(2, use_graph=True)
+            if tune:
+                r = ens.autotune(sort_intervals=(0, 2), warmup_steps=1, measured_steps=2, sort_trial_steps=3)
+                say(f"  autotune: {r}")
+                assert r["block_size"] in (128, 256, 512) and r["particles_per_thread"] in (1, 2, 4)
+                assert r["sort_interval"] in (0, 2) and r["step_ms"] > 0.0
+            ctr = ens.step(3, use_graph=True)
+            outs.append(([ens.download_member(m) for m in range(len(CAPS))], ctr,
+                         [ens.download_grid_member(m) for m in range(len(CAPS))]))
+    (pa, ca, ga), (pb, cb, gb) = outs
+    assert np.array_equal(ca, cb)
+    for m in range(len(CAPS)):
+        assert_particles_equal(pa[m], pb[m], f"member {m} tuned vs untuned")
+        assert np.array_equal(ga[m][0], gb[m][0]) and np.array_equal(ga[m][1], gb[m][1])
+    with build_ensemble(fp, small) as ens:
+        with pytest.raises(fp.FpError) as e:
+            ens.autotune()
+        assert e.value.code == fp.FP_E_STATE
