@@ -736,7 +736,8 @@ struct fp_adj {
     uint64_t *d_offs = nullptr;
     float *d_q = nullptr, *d_r = nullptr, *d_c = nullptr;
 
-    fp::Soa work, work2;               // forward state; compaction target and backward scratch
+    fp::Soa work, work2;               // forward states between checkpoints
+    fp::Soa final_state;               // where the last forward's final state is
     std::vector<uint32_t> ckpt_steps;  // steps whose state is kept (0, K, 2K, ..., n_steps)
     std::vector<fp::Soa> ckpt;
     std::vector<fp::Soa> tape;         // K - 1 sets
@@ -969,7 +970,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
         return amalloc(&s.base, total * fp::F_COUNT);
     };
     C_HIP(soa_alloc(e->work));
-    C_HIP(soa_alloc(e->work2));
+    if (e->compact) C_HIP(soa_alloc(e->work2));  // the compacting loop alternates two sets
     e->ckpt.resize(e->ckpt_steps.size());
     for (auto &s : e->ckpt) C_HIP(soa_alloc(s));
     e->tape.resize(d.checkpoint_every > 1 ? d.checkpoint_every - 1u : 0u);
@@ -989,7 +990,8 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     C_HIP(amalloc(&e->av, e->nxy));
     C_HIP(amalloc(&e->aos_stage, total));
     // zero every particle set once so that fields the step never writes read as zero
-    for (fp::Soa *s : {&e->work, &e->work2}) C_HIP(hipMemsetAsync(s->base, 0, total * fp::F_COUNT * 4u, e->stream));
+    for (fp::Soa *s : {&e->work, &e->work2})
+        if (s->base) C_HIP(hipMemsetAsync(s->base, 0, total * fp::F_COUNT * 4u, e->stream));
     for (auto &s : e->ckpt) C_HIP(hipMemsetAsync(s.base, 0, total * fp::F_COUNT * 4u, e->stream));
     for (auto &s : e->tape) C_HIP(hipMemsetAsync(s.base, 0, total * fp::F_COUNT * 4u, e->stream));
     C_HIP(hipMemsetAsync(e->node_rec, 0xff, e->n_nodes * 4u, e->stream));
@@ -1126,69 +1128,81 @@ int fp_adj_forward(fp_adj *e, const fp_step_params *p, const float *q, const flo
             e->met_base, e->npts, e->d.met_nz, b, K ? e->pert_u : nullptr, K ? e->pert_v : nullptr,
             e->met_packed + (uint64_t)b * e->npts * 2u);
     A_HIP(e, hipGetLastError());
-    // sources
-    fp::init_sources_kernel<<<fp::grid_for(e->n), 256, 0, st>>>(e->work, (uint32_t)e->n, e->src_of, e->d_counts,
-                                                               e->d_q, e->d_r, e->d.met_nx, e->d.met_ny);
+    // sources: the state at step 0 is the first checkpoint
+    fp::init_sources_kernel<<<fp::grid_for(e->n), 256, 0, st>>>(e->ckpt[0], (uint32_t)e->n, e->src_of,
+                                                               e->d_counts, e->d_q, e->d_r, e->d.met_nx,
+                                                               e->d.met_ny);
     fp::fill_f64_kernel<<<fp::grid_for(std::max<uint64_t>(e->n_rec, 1)), 256, 0, st>>>(e->rec_val, e->n_rec, 0.0);
     A_HIP(e, hipGetLastError());
-    A_HIP(e, copy_state(e, e->ckpt[0], e->work, e->n));
     const fp::ObsArgs oa = obs_args(e);
     e->step_n[0] = (uint32_t)e->n;
     for (int cc = 0; cc < 4; ++cc) e->step_ctr[0][cc] = p->counter[cc];
+    // A step reads the state where the previous one left it and writes it
+    // where it is needed next, so no state is copied: a step whose end is a
+    // checkpoint writes into that checkpoint, the step after it reads from
+    // there (launch_step_out_of_place), the others step in place.
     // With compaction after every step (timeloop.rs:1185-1197), the records
     // are not moved by the compaction itself: step k + 1 reads survivor j of
     // step k through the saved map (launch_step_gathered) and writes it to
     // slot j, which is the state and the Philox slot a moved record would have.
-    // The gathered step writes the words a step changes; every other word of
-    // a record is zero here in both sets (sources start with zero words, the
-    // sets are zeroed at creation and no kernel writes those words), so the
-    // records are whole.  Only the last compaction moves the records, so that
-    // the final state is the compacted one.
+    // Only the last compaction moves the records, so that the final state is
+    // the compacted one.  The out-of-place and gathered steps write the words
+    // a step changes; every other word of a record is zero here in every set
+    // (sources start with zero words, the sets are zeroed at creation and no
+    // kernel writes those words), so the records are whole.
+    fp::Soa cur = e->ckpt[0];
     uint64_t moff = 0;
     for (uint32_t k = 0; k < e->d.n_steps; ++k) {
         const uint32_t nk = e->step_n[k];
-        const bool gathered = e->compact && k > 0;
-        fp::StepArgs a = step_args(e, k, gathered ? e->work2 : e->work);
-        if (gathered) {
-            a.gather_map = e->maps + e->map_off[k - 1u];
-            a.src = e->work;
-            A_HIP(e, fp::launch_step_gathered(a, false, st));
-            std::swap(e->work, e->work2);
-        } else {
-            A_HIP(e, fp::launch_step(a, 256, st));
-        }
-        // e->work: the state after step k (before its compaction)
-        if (observed(e, k + 1u) && nk)
-            fp::obs_deposit_kernel<<<fp::grid_for(nk), 256, 0, st>>>(e->work, nk, oa, e->rec_val);
-        uint32_t next = nk;
+        const int j = ckpt_index(e, k + 1u);
         const bool last = k + 1u == e->d.n_steps;
+        fp::Soa dst;
+        if (e->compact) dst = cur.base == e->work.base ? e->work2 : e->work;
+        else dst = j >= 0 ? e->ckpt[j] : e->work;
+        fp::StepArgs a = step_args(e, k, dst);
+        if (dst.base == cur.base) {
+            A_HIP(e, fp::launch_step(a, 256, st));
+        } else if (e->compact && k > 0) {
+            a.gather_map = e->maps + e->map_off[k - 1u];
+            a.src = cur;
+            A_HIP(e, fp::launch_step_gathered(a, false, st));
+        } else {
+            a.src = cur;
+            A_HIP(e, fp::launch_step_out_of_place(a, st));
+        }
+        cur = dst;
+        // cur: the state after step k (before its compaction)
+        if (observed(e, k + 1u) && nk)
+            fp::obs_deposit_kernel<<<fp::grid_for(nk), 256, 0, st>>>(cur, nk, oa, e->rec_val);
+        uint32_t next = nk;
         if (e->compact) {
             e->map_off[k] = moff;
-            if (last)
-                A_HIP(e, fp::launch_compact(e->work, e->work2, nk, e->block_counts, e->maps + moff, e->active_dev,
-                                            st));
-            else
-                A_HIP(e, fp::launch_survivor_map(e->work, nk, e->block_counts, e->maps + moff, e->active_dev, st));
+            if (last) {
+                const fp::Soa out = cur.base == e->work.base ? e->work2 : e->work;
+                A_HIP(e, fp::launch_compact(cur, out, nk, e->block_counts, e->maps + moff, e->active_dev, st));
+                cur = out;
+            } else {
+                A_HIP(e, fp::launch_survivor_map(cur, nk, e->block_counts, e->maps + moff, e->active_dev, st));
+            }
             A_HIP(e, hipMemcpyAsync(e->active_host, e->active_dev, 4, hipMemcpyDeviceToHost, st));
             A_HIP(e, hipStreamSynchronize(st));
             next = *e->active_host;
             moff += next;
-            if (last) std::swap(e->work, e->work2);
         }
         e->step_n[k + 1u] = next;
         fp::counter_advance(e->step_ctr[k].data(), fp::counter_per_step(nk, p->n_substeps),
                             e->step_ctr[k + 1u].data());
-        const int j = ckpt_index(e, k + 1u);
-        if (j >= 0) {
-            if (e->compact && !last) {
+        if (j >= 0 && e->compact) {
+            if (!last) {
                 if (next)
-                    fp::gather_state_kernel<<<fp::grid_for(next), 256, 0, st>>>(e->work, e->ckpt[j],
+                    fp::gather_state_kernel<<<fp::grid_for(next), 256, 0, st>>>(cur, e->ckpt[j],
                                                                               e->maps + e->map_off[k], next);
             } else {
-                A_HIP(e, copy_state(e, e->ckpt[j], e->work, next));
+                A_HIP(e, copy_state(e, e->ckpt[j], cur, next));
             }
         }
     }
+    e->final_state = cur;
     A_HIP(e, hipGetLastError());
     A_HIP(e, hipEventRecord(e->e1, st));
     e->val_host.assign(e->n_rec, 0.0);
@@ -1248,8 +1262,9 @@ int fp_adj_backward(fp_adj *e, double *grad_q, double *grad_r, double *grad_c)
                 a.src = prev;
                 A_HIP(e, fp::launch_step_gathered(a, true, st));
             } else {
-                A_HIP(e, copy_state(e, cur, prev, e->step_n[k - 1u]));
-                A_HIP(e, fp::launch_step(step_args(e, k - 1u, cur), 256, st));
+                fp::StepArgs a = step_args(e, k - 1u, cur);
+                a.src = prev;
+                A_HIP(e, fp::launch_step_out_of_place(a, st));
             }
         }
         for (uint32_t k = k1; k-- > k0;) {
@@ -1312,7 +1327,7 @@ int fp_adj_download_state(fp_adj *e, fp_particle *out, uint64_t n)
     A_CHECK(e, n <= e->n, "more records than particles");
     A_STATE(e, e->forward_done, "fp_adj_download_state before fp_adj_forward");
     if (n == 0) return FP_OK;
-    A_HIP(e, fp::launch_soa_to_aos(e->work, e->aos_stage, n, e->stream));  // the backward pass leaves it
+    A_HIP(e, fp::launch_soa_to_aos(e->final_state, e->aos_stage, n, e->stream));  // the backward leaves it
     A_HIP(e, hipMemcpyAsync(out, e->aos_stage, n * sizeof(fp_particle), hipMemcpyDeviceToHost, e->stream));
     A_HIP(e, hipStreamSynchronize(e->stream));
     return FP_OK;

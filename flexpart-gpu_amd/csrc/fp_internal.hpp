@@ -256,6 +256,8 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
 // one step of a.n slots through a.gather_map (block 256, a.ppt rows); the
 // particle steps at its destination slot, or at its source slot
 hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st);
+// one step of slots [0, a.n) of a.src into the same slots of a.s (stepped words)
+hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st);
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
                            uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st);
 hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,

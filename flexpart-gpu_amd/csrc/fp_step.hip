@@ -52,7 +52,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         const bool inb = i < a.n;
         // the slot the state is read from
         uint64_t g = i;
-        if (GATHER != kGatherNone && inb) g = a.gather_map[i];
+        if (GATHER != kGatherNone && inb && a.gather_map) g = a.gather_map[i];
         const Soa sl = GATHER != kGatherNone ? a.src : s;
         // the slot the particle occupies during the step
         const uint64_t at = GATHER == kGatherFromSlot ? g : i;
@@ -327,7 +327,7 @@ template <int GATHER>
 hipError_t launch_gathered(const StepArgs &a, hipStream_t st)
 {
     if (a.n == 0) return hipSuccess;
-    if (!a.gather_map || a.src.base == a.s.base) return hipErrorInvalidValue;
+    if ((GATHER == kGatherFromSlot && !a.gather_map) || a.src.base == a.s.base) return hipErrorInvalidValue;
     const uint64_t per_block = 256ull * a.ppt;
     const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
     if (a.n_substeps == 4u) step_kernel<256, 4, GATHER><<<grid, 256, 0, st>>>(a);
@@ -337,7 +337,18 @@ hipError_t launch_gathered(const StepArgs &a, hipStream_t st)
 
 hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st)
 {
+    if (!a.gather_map) return hipErrorInvalidValue;
     return at_source_slot ? launch_gathered<kGatherFromSlot>(a, st) : launch_gathered<kGatherToSlot>(a, st);
+}
+
+// The step of slots [0, a.n) of a.src written to the same slots of a.s (the
+// stepped words only, as in launch_step_gathered): a step and a copy of the
+// state in one pass.
+hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st)
+{
+    StepArgs b = a;
+    b.gather_map = nullptr;
+    return launch_gathered<kGatherToSlot>(b, st);
 }
 
 template <uint32_t BLOCK>
