@@ -122,6 +122,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread (0 = autotuned)")
     ap.add_argument("--tune-at", type=int, default=72, help="autotune on the state after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="main workload: skip the 2M-particle adjoint run reported beside the step rate")
     ap.add_argument("--loss-fraction", type=float, default=0.3,
                     help="stress workload: a particle is lost once its mass falls below this "
                          "fraction of the common initial particle mass")
@@ -294,6 +296,18 @@ def main(argv=None) -> int:
     if not args.no_cpu_baseline:
         threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
         result["cpu_baseline"] = cpu_baseline(w, threads)
+    if not args.no_adjoint:
+        # the metric's second part: the 2M-particle adjoint run (72 steps)
+        adj = adjoint_result(72)
+        emis = [c["rel_err"] for c in adj["config"]["fd_checks"]["emission"]]
+        result["config"]["adjoint"] = {
+            "workload": adj["config"]["workload"],
+            "adjoint_over_forward": adj["value"],
+            "forward_ms": adj["config"]["forward_ms"],
+            "backward_ms": adj["config"]["backward_ms"],
+            "emission_fd_rel_err_max": max(emis),
+            "fd_relative_step": 1e-3,
+        }
     print(json.dumps(result), flush=True)
     return 0
 
@@ -503,6 +517,11 @@ def bench_ensemble(args) -> int:
 
 
 def bench_adjoint(args) -> int:
+    print(json.dumps(adjoint_result(args.steps)), flush=True)
+    return 0
+
+
+def adjoint_result(n_steps: int) -> dict:
     """The differentiable run: 2M particles from 1024 sources (lognormal
     emissions, uniform-random positions, heights U[10, 500] m), 12 h (72 steps
     of 600 s) on the 360 x 181 x 60 global met with 3 time levels, 64 wind
@@ -518,7 +537,6 @@ def bench_adjoint(args) -> int:
     fp = _package()
     import flexpart_gpu_amd.workloads as W
 
-    n_steps = args.steps
     c = W.adjoint_case()
     nx, ny, nz = c.met.shape
     obs_nz, obs_dz = 11, 500.0
@@ -612,8 +630,7 @@ def bench_adjoint(args) -> int:
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(result), flush=True)
-    return 0
+    return result
 
 
 def bench_stress(args) -> int:
