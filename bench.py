@@ -532,7 +532,7 @@ def adjoint_result(n_steps: int) -> dict:
     pair) and checks gradients against central differences with relative step
     1e-3."""
     import numpy as np
-    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+    import torch  # noqa: F401  (torch's HIP runtime loads before the library, as in the tests)
 
     fp = _package()
     import flexpart_gpu_amd.workloads as W
@@ -643,7 +643,7 @@ def bench_stress(args) -> int:
     emission adjoint through the saved survivor maps (HIP events on the
     library stream; median of 2 after a warm-up pair)."""
     import numpy as np
-    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+    import torch  # noqa: F401  (torch's HIP runtime loads before the library, as in the tests)
 
     fp = _package()
     import flexpart_gpu_amd.workloads as W
@@ -750,7 +750,7 @@ def bench_pbl(args) -> int:
     oracle (one thread) on the same grid, the reference's own per-step CPU
     work (interpolate_surface_fields_linear + compute_pbl_parameters_from_met)."""
     import numpy as np
-    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+    import torch  # noqa: F401  (torch's HIP runtime loads before the library, as in the tests)
 
     fp = _package()
     import flexpart_gpu_amd.workloads as W
@@ -827,7 +827,7 @@ def bench_output(args) -> int:
     import tempfile
 
     import numpy as np
-    import torch  # noqa: F401  (same HIP runtime as the tests)
+    import torch  # noqa: F401  (torch's HIP runtime loads before the library, as in the tests)
 
     fp = _package()
     import flexpart_gpu_amd.workloads as W
@@ -926,7 +926,7 @@ def bench_met(args) -> int:
     memory on the copy stream while the 36 steps of the current bracket run.
     CPU baseline: the oracle's transform (one thread) on the same snapshot."""
     import numpy as np
-    import torch  # noqa: F401  (same HIP runtime as the tests; see INTEGRATION.md)
+    import torch  # noqa: F401  (torch's HIP runtime loads before the library, as in the tests)
 
     fp = _package()
     import flexpart_gpu_amd.workloads as W

@@ -1031,7 +1031,7 @@ void orc_pbl_from_surface(uint64_t n, const float *const s0[ORC_SURF_COUNT], con
 }
 
 /* ---------------------------------------------------------------------- */
-/* Concentration output: gpu/gridding.rs:194-219, concentration-gridding.md */
+/* Concentration output: gpu/gridding.rs:194-219, docs/science/concentration-gridding.md */
 /* ---------------------------------------------------------------------- */
 
 /* dx_meters_at_latitude (coords/mod.rs:156-158), f32 with R_EARTH = 6.371e6
