@@ -153,13 +153,17 @@ def cic_values(c, parts, table, acc):
         np.add.at(acc, rec[ok], (m[ok] * w[ok]).astype(f32).astype(np.float64))
 
 
-def test_forward_is_the_production_step(fp, orc, case):
+@pytest.mark.parametrize("ckpt", [4, 1, 8])
+def test_forward_is_the_production_step(fp, orc, case, ckpt):
+    """Checkpoints every 4 steps, every step (each step writes a new
+    checkpoint) and beyond the run (only the last step writes one)."""
     from test_gpu_parity import oracle_params
     c = case
     n_steps, obs_first = 6, 4
-    say(f"adjoint forward vs oracle: {len(COUNTS)} sources, {sum(COUNTS)} particles, {n_steps} steps")
+    say(f"adjoint forward vs oracle: {len(COUNTS)} sources, {sum(COUNTS)} particles, {n_steps} steps, "
+        f"checkpoint every {ckpt}")
     nodes = all_nodes(c)
-    with make_adj(fp, c, n_steps, obs_first) as adj:
+    with make_adj(fp, c, n_steps, obs_first, ckpt=ckpt) as adj:
         adj.set_receptors(nodes)
         cfg = cfg_of(fp, c)
         vals, loss = adj.forward(cfg, c.emission, c.position, c.coeff)
@@ -594,3 +598,23 @@ def test_compacting_backward_matches_central_differences(fp, case):
         assert counts[1] == sum(COUNTS) - int(np.sum(c.counts[dead])) == counts[-1]
         assert np.all(gq[dead] == 0.0) and np.all(np.abs(fd[dead]) <= 1e-6 * np.max(np.abs(fd)))
         np.testing.assert_allclose(gq, fd, rtol=1e-3, atol=1e-4 * np.max(np.abs(fd)))
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_gradients_do_not_depend_on_the_checkpoint_interval(fp, case, receptors, compact):
+    """The backward recomputes every state between checkpoints exactly, so the
+    gradients are those of any other checkpoint interval (up to the order of
+    the f64 atomic sums)."""
+    c = case
+    nodes, obs = receptors
+    cfg = lossy_cfg(fp, c) if compact else cfg_of(fp, c)
+    say(f"gradients vs checkpoint interval (compact={compact})")
+    grads = []
+    for ckpt in (1, 3, 4, 8):
+        with make_adj(fp, c, ckpt=ckpt, compact=compact) as adj:
+            adj.set_receptors(nodes, obs)
+            adj.forward(cfg, c.emission, c.position, c.coeff)
+            grads.append(adj.backward())
+    for g in grads[1:]:
+        for a, b in zip(g, grads[0]):
+            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * max(1.0, float(np.max(np.abs(b)))))
