@@ -266,6 +266,26 @@ def test_compaction_matches_oracle(fp, orc, small, pattern):
         assert_particles_equal(ctx.download_particles(), oout, "compaction")
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 2047, 2048, 2049, 4097])
+def test_compaction_tile_edges_match_oracle(fp, orc, small, n):
+    """Sizes around the wave (64) and the scatter's tile (2048 slots): a
+    partial last tile, a single slot, exact multiples; then a compaction of
+    the already compacted store (dispatch narrowed, possibly to zero)."""
+    say(f"compaction tile edges: n={n}")
+    p = small.particles[:n].copy()
+    p["flags"] = (np.random.default_rng(n).uniform(size=n) < 0.5).astype(np.uint32)
+    with fp.GpuTimestep(n, small.met.shape, n_species=4) as ctx:
+        ctx.upload_store(p)
+        a, smap = ctx.compact()
+        oa, oout, omap = orc.compact(p)
+        assert a == oa and np.array_equal(smap, omap)
+        assert_particles_equal(ctx.download_particles(), oout, f"compaction n={n}")
+        a2, smap2 = ctx.compact()
+        oa2, oout2, omap2 = orc.compact(oout[:oa])
+        assert a2 == oa2 == oa and np.array_equal(smap2, omap2)
+        assert_particles_equal(ctx.download_particles(n=oa), oout2, f"second compaction n={n}")
+
+
 def test_compaction_then_steps_then_release_into_free_slots(fp, orc, small):
     say("compaction in a loop: step, compact, release, step")
     want = small.particles.copy()
