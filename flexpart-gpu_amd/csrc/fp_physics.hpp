@@ -568,36 +568,30 @@ __device__ __forceinline__ void grid_lds_flush(GridLds &t, const GridArgs &g)
     }
 }
 
-// Wave-level pre-reduction of same-cell contributions into the block's
-// table.  Every lane of the wave must call this (lanes without a
-// contribution pass valid = false).  Each pass takes the lowest pending
-// lane's cell (v_readlane with a wave-uniform index), counts the lanes with
-// that cell and sums their masses with a DPP reduction; the totals of pass j
-// are kept in lane j, and the distinct cells of the wave then enter the
-// table in parallel, one per lane.
+// Wave-level pre-reduction into the block's table.  Every lane of the wave
+// must call this (lanes without a contribution pass valid = false).  The
+// lowest contributing lane's cell is read with v_readlane, the lanes with
+// that cell are counted and their masses summed with a DPP reduction, and
+// that total enters the table once; every other contribution enters on its
+// own, in parallel (the table is the block-level reduction).  One pass takes
+// a plume's hot cell out of the table's atomics; iterating the passes over
+// every distinct cell of a spread-out wave cost more instructions than the
+// table inserts they saved.
 __device__ __forceinline__ void grid_reduce_wave(GridLds &t, const GridArgs &g, bool valid, uint32_t cell,
                                                  uint32_t q)
 {
-    uint64_t pending = __ballot(valid);
+    const uint64_t pending = __ballot(valid);
+    if (!pending) return;
     const uint32_t lane = __lane_id();
-    uint32_t my_cell = 0u, my_cnt = 0u, my_mass = 0u;
-    uint32_t j = 0;
-    while (pending) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane((int)cell, (int)leader);
-        const bool mine = valid && cell == lc;
-        const uint64_t same = __ballot(mine);
-        const uint32_t v = wave_sum_to_lane63(mine ? q : 0u);
-        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(kWave - 1u));
-        if (lane == j) {
-            my_cell = lc;
-            my_cnt = (uint32_t)__popcll(same);
-            my_mass = tot;
-        }
-        ++j;
-        pending &= ~same;
-    }
-    if (lane < j) grid_lds_add(t, g, my_cell, my_cnt, my_mass);
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane((int)cell, (int)leader);
+    const bool mine = valid && cell == lc;
+    const uint64_t same = __ballot(mine);
+    const uint32_t v = wave_sum_to_lane63(mine ? q : 0u);
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(kWave - 1u));
+    const bool lead = lane == leader;
+    if (lead || (valid && !mine))
+        grid_lds_add(t, g, lead ? lc : cell, lead ? (uint32_t)__popcll(same) : 1u, lead ? tot : q);
 }
 
 // XCD-aware chunking: the hardware deals blocks round-robin over the 8
