@@ -447,15 +447,24 @@ def bench_ensemble(args) -> int:
 
     # (a') the plain loop as a per-member driver runs it: member after member,
     # each step followed by a stream synchronisation (the point where a
-    # driver reads back or inspects its state)
-    t0 = time.perf_counter()
-    for h, p, nxt in params:
-        for k in range(args.steps):
-            p.alpha = alpha[min(k, window)]
-            if lib.fp_step(h, C.byref(p), nxt) != 0 or lib.fp_sync(h) != 0:
-                raise RuntimeError(lib.fp_last_error(h).decode())
-            C.memmove(p.counter, nxt, 16)
-    t_serial = time.perf_counter() - t0
+    # driver reads back or inspects its state); timed with the ensemble's
+    # tuned launch shape and with the library's default shape (more waves for
+    # a small member), and the faster of the two is the baseline
+    def serial_loop():
+        t0 = time.perf_counter()
+        for h, p, nxt in params:
+            for k in range(args.steps):
+                p.alpha = alpha[min(k, window)]
+                if lib.fp_step(h, C.byref(p), nxt) != 0 or lib.fp_sync(h) != 0:
+                    raise RuntimeError(lib.fp_last_error(h).decode())
+                C.memmove(p.counter, nxt, 16)
+        return time.perf_counter() - t0
+
+    t_serial_tuned = serial_loop()
+    for c in loop:
+        c.set_launch(256, 1)
+    t_serial_default = serial_loop()
+    t_serial = min(t_serial_tuned, t_serial_default)
 
     # (b) ensemble, one direct launch per step; (c) the captured graph
     ens.configure(cfg, keys=keys, counters=ens_ctr, bracket_s=bracket_s)
@@ -499,6 +508,8 @@ def bench_ensemble(args) -> int:
             "autotune_step_ms_with_sort": tuned["step_ms_with_sort"],
             "loop_streams_particle_steps_per_s": ps / t_loop,
             "loop_serial_particle_steps_per_s": ps / t_serial,
+            "loop_serial_tuned_shape_particle_steps_per_s": ps / t_serial_tuned,
+            "loop_serial_default_shape_particle_steps_per_s": ps / t_serial_default,
             "direct_particle_steps_per_s": ps / t_direct,
             "graph_particle_steps_per_s": ps / t_graph,
             "speedup_graph_vs_loop_serial": t_serial / t_graph,
