@@ -431,19 +431,30 @@ def bench_ensemble(args) -> int:
     for m in range(M):
         p = fp.StepConfig(**{**cfg.__dict__, "key": tuple(keys[m]), "counter": loop_ctr[m]}).to_c()
         params.append((loop[m]._h, p, (C.c_uint32 * 4)()))
+    def streams_loop():
+        for c in loop:
+            c.sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            a = alpha[min(k, window)]
+            for h, p, nxt in params:
+                p.alpha = a
+                if lib.fp_step(h, C.byref(p), nxt) != 0:
+                    raise RuntimeError(lib.fp_last_error(h).decode())
+                C.memmove(p.counter, nxt, 16)
+        for c in loop:
+            c.sync()
+        return time.perf_counter() - t0
+
+    # timed with the ensemble's tuned launch shape and with the library's
+    # default shape; the faster of the two is reported
+    t_loop_tuned = streams_loop()
     for c in loop:
-        c.sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        a = alpha[min(k, window)]
-        for h, p, nxt in params:
-            p.alpha = a
-            if lib.fp_step(h, C.byref(p), nxt) != 0:
-                raise RuntimeError(lib.fp_last_error(h).decode())
-            C.memmove(p.counter, nxt, 16)
+        c.set_launch(256, 1)
+    t_loop_default = streams_loop()
+    t_loop = min(t_loop_tuned, t_loop_default)
     for c in loop:
-        c.sync()
-    t_loop = time.perf_counter() - t0
+        c.set_launch(block, ppt)
 
     # (a') the plain loop as a per-member driver runs it: member after member,
     # each step followed by a stream synchronisation (the point where a
@@ -507,6 +518,8 @@ def bench_ensemble(args) -> int:
             "autotune_step_ms": tuned["step_ms"],
             "autotune_step_ms_with_sort": tuned["step_ms_with_sort"],
             "loop_streams_particle_steps_per_s": ps / t_loop,
+            "loop_streams_tuned_shape_particle_steps_per_s": ps / t_loop_tuned,
+            "loop_streams_default_shape_particle_steps_per_s": ps / t_loop_default,
             "loop_serial_particle_steps_per_s": ps / t_serial,
             "loop_serial_tuned_shape_particle_steps_per_s": ps / t_serial_tuned,
             "loop_serial_default_shape_particle_steps_per_s": ps / t_serial_default,
