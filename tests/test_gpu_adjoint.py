@@ -618,3 +618,95 @@ def test_gradients_do_not_depend_on_the_checkpoint_interval(fp, case, receptors,
     for g in grads[1:]:
         for a, b in zip(g, grads[0]):
             np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * max(1.0, float(np.max(np.abs(b)))))
+
+
+def test_compacting_24h_stress_run_matches_the_oracle_sequence(fp, orc, workloads):
+    """The deposition stress configuration at 200k particles on the 1 deg
+    met: 144 steps with dry deposition, precipitation scavenging in patches
+    and the loss rule, compacted after every step through the gathered steps
+    (records never moved until the last compaction).  Active counts, the
+    final compacted records and receptor sums equal the oracle's step +
+    compaction sequence."""
+    import os
+    from test_gpu_parity import oracle_params
+    c = workloads.stress_case(200_000, 512, deg=1.0)
+    n_steps, obs_first = 144, 133
+    m0 = float(np.median(c.emission / c.counts.astype(np.float32)))
+    cfg = fp.StepConfig(dt_seconds=c.dt_seconds, x_scale=c.x_scale, y_scale=c.y_scale, key=c.key,
+                        dry_mode=1, dry_vd=1e-2, wet_mode=3, wet_a=1e-4, wet_b=0.8, wet_fraction=1.0,
+                        loss_mode=1, loss_min_mass=0.3 * m0)
+    nx, ny, _ = c.met.shape
+    # receptors on the lattice nodes around the first 300 sources, three levels
+    src = np.floor(c.position[:300, :2]).astype(np.int64)
+    nodes = np.unique(np.array([(i + di, j + dj, l) for i, j in src for di in (0, 1) for dj in (0, 1)
+                                for l in (0, 1, 2) if i + di < nx and j + dj < ny]), axis=0)
+    say(f"compacting stress run vs oracle: {int(c.counts.sum())} particles, {n_steps} steps")
+    adj = fp.GpuAdjoint(c.counts, c.met.shape, n_met_levels=2, n_modes=0, obs_nz=11, obs_dz_m=500.0,
+                        n_steps=n_steps, obs_first_step=obs_first, checkpoint_every=8, compact=True,
+                        emission_only=True)
+    with adj:
+        adj.upload_met(c.met.levels, c.met.level_heights, c.met.seconds_between_levels)
+        adj.upload_pbl(c.pbl.ustar, c.pbl.wstar, c.pbl.hmix, c.pbl.oli, c.pbl.precip)
+        adj.set_receptors(nodes)
+        vals, _ = adj.forward(cfg, c.emission, c.position)
+        counts = adj.active_counts()
+        got = adj.download_state(int(counts[-1]))
+    parts = initial_particles(fp, c)
+    (u0, v0, w0), (u1, v1, w1) = c.met.levels
+    met = dict(shape=c.met.shape, u0=u0, v0=v0, w0=w0, u1=u1, v1=v1, w1=w1, level_heights=c.met.level_heights)
+    T = int(c.met.seconds_between_levels)
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    table = np.full(nx * ny * 11, -1, np.int64)
+    table[(nodes[:, 0] * ny + nodes[:, 1]) * 11 + nodes[:, 2]] = np.arange(len(nodes))
+    want_vals = np.zeros(len(nodes))
+    counter = tuple(cfg.counter)
+    n_cur = len(parts)
+    want_counts = [n_cur]
+    for k in range(n_steps):
+        t = int(k * c.dt_seconds)
+        cfg.alpha = fp.interpolation_alpha(0, T, t)
+        cfg.counter = counter
+        live = parts[:n_cur].copy()
+        p = oracle_params(c, cfg)
+        p["n_threads"] = threads
+        counter = tuple(int(v) for v in orc.step(live, met, pbl_dict(c.pbl), p))
+        if k + 1 >= obs_first:
+            cic_values_lattice(c, live, table, want_vals, 11, 500.0)
+        n_cur, out, _ = orc.compact(live)
+        parts[: len(out)] = out
+        want_counts.append(n_cur)
+    say(f"  active {want_counts[0]} -> {want_counts[-1]}")
+    assert list(counts) == want_counts
+    assert want_counts[-1] < 0.7 * want_counts[0], "the run must lose particles"
+    assert got.tobytes() == parts[: want_counts[-1]].tobytes()
+    assert np.count_nonzero(want_vals) > 0
+    np.testing.assert_allclose(vals, want_vals, rtol=1e-9, atol=1e-12 * max(1.0, float(np.max(want_vals))))
+
+
+def cic_values_lattice(c, parts, table, acc, obs_nz, obs_dz):
+    """cic_values for an observation lattice of obs_nz levels obs_dz apart."""
+    nx, ny, _ = c.met.shape
+    f32 = np.float32
+    act = (parts["flags"] & 1) != 0
+    x = (parts["cell_x"].astype(f32) + parts["pos_x"])[act]
+    y = (parts["cell_y"].astype(f32) + parts["pos_y"])[act]
+    z = parts["pos_z"][act]
+    m = parts["mass"][act, 0]
+    xc = np.clip(x, f32(0), f32(nx - 1))
+    yc = np.clip(y, f32(0), f32(ny - 1))
+    zc = (np.clip(z, f32(0), f32((obs_nz - 1) * obs_dz)) / f32(obs_dz)).astype(f32)
+    i0 = np.minimum(np.floor(xc).astype(np.int64), nx - 2)
+    j0 = np.minimum(np.floor(yc).astype(np.int64), ny - 2)
+    l0 = np.minimum(np.floor(zc).astype(np.int64), obs_nz - 2)
+    fx = (xc - i0.astype(f32)).astype(f32)
+    fy = (yc - j0.astype(f32)).astype(f32)
+    fz = (zc - l0.astype(f32)).astype(f32)
+    for k in range(8):
+        wx = fx if k & 1 else (f32(1) - fx)
+        wy = fy if k & 2 else (f32(1) - fy)
+        wz = fz if k & 4 else (f32(1) - fz)
+        w = ((wx * wy).astype(f32) * wz).astype(f32)
+        node = ((i0 + (k & 1)) * ny + j0 + ((k >> 1) & 1)) * obs_nz + l0 + (k >> 2)
+        rec = table[node]
+        ok = rec >= 0
+        np.add.at(acc, rec[ok], (m[ok] * w[ok]).astype(f32).astype(np.float64))
