@@ -22,8 +22,9 @@
  *   fpm_logf      <= 1 ulp         (atanh series in s = f/(2+f), 4 terms)
  *   fpm_sincosf   <= 1 ulp on |x| <= 2*pi, the Box-Muller angle range
  *                 (3-part Cody-Waite reduction by pi/2, Taylor kernels)
- *   fpm_powf      correctly rounded except within ~1e-7 ulp of a midpoint
- *                 (exp(y*log(x)) evaluated in binary64, rounded once)
+ *   fpm_powf      correctly rounded except within ~1e-8 ulp of a midpoint
+ *                 (exp(y*log(x)) evaluated in binary64 with table-driven
+ *                 reductions, rounded once)
  */
 #ifndef FP_MATH_H
 #define FP_MATH_H
@@ -151,58 +152,142 @@ FPM_FN void fpm_sincosf(float x, float *sin_out, float *cos_out)
 
 /* ---- binary64 kernels used by fpm_powf ------------------------------- */
 
-/* log(x) for a positive, finite, normal binary64 x; ~1e-16 relative. */
+/* Tables of the binary64 kernels (constants of the device code on the GPU).
+ * FPM_LOG_INVC[i] = 1/c_i rounded to binary64, c_i = 1 + (i + 1/2)/128, the
+ * centre of the i-th 1/128 mantissa interval; FPM_LOG_T[i] = -log of that
+ * rounded reciprocal, to binary64 (so that log m = T_i + log(m * invc_i)
+ * exactly); FPM_EXP2_J[j] = 2^(j/32) rounded to binary64.  Generated with
+ * 60-digit decimal arithmetic. */
+#if defined(__cplusplus)
+#define FPM_TABLE static constexpr
+#else
+#define FPM_TABLE static const
+#endif
+FPM_TABLE double FPM_LOG_INVC[128] = {
+    0.9961089494163424, 0.9884169884169884, 0.9808429118773946, 0.973384030418251,
+    0.9660377358490566, 0.9588014981273408, 0.9516728624535316, 0.9446494464944649,
+    0.9377289377289377, 0.9309090909090909, 0.924187725631769, 0.9175627240143369,
+    0.9110320284697508, 0.9045936395759717, 0.8982456140350877, 0.89198606271777,
+    0.8858131487889274, 0.8797250859106529, 0.8737201365187713, 0.8677966101694915,
+    0.8619528619528619, 0.8561872909698997, 0.8504983388704319, 0.8448844884488449,
+    0.839344262295082, 0.8338762214983714, 0.8284789644012945, 0.8231511254019293,
+    0.8178913738019169, 0.8126984126984127, 0.807570977917981, 0.8025078369905956,
+    0.7975077881619937, 0.7925696594427245, 0.7876923076923077, 0.7828746177370031,
+    0.7781155015197568, 0.7734138972809668, 0.7687687687687688, 0.764179104477612,
+    0.7596439169139466, 0.7551622418879056, 0.750733137829912, 0.7463556851311953,
+    0.7420289855072464, 0.7377521613832853, 0.7335243553008596, 0.7293447293447294,
+    0.7252124645892352, 0.7211267605633803, 0.7170868347338936, 0.713091922005571,
+    0.7091412742382271, 0.7052341597796143, 0.7013698630136986, 0.6975476839237057,
+    0.6937669376693767, 0.6900269541778976, 0.6863270777479893, 0.6826666666666666,
+    0.6790450928381963, 0.6754617414248021, 0.6719160104986877, 0.6684073107049608,
+    0.6649350649350649, 0.661498708010336, 0.6580976863753213, 0.6547314578005116,
+    0.6513994910941476, 0.6481012658227848, 0.6448362720403022, 0.6416040100250626,
+    0.6384039900249376, 0.6352357320099256, 0.6320987654320988, 0.628992628992629,
+    0.6259168704156479, 0.6228710462287105, 0.6198547215496368, 0.6168674698795181,
+    0.6139088729016786, 0.6109785202863962, 0.6080760095011877, 0.6052009456264775,
+    0.6023529411764705, 0.5995316159250585, 0.5967365967365967, 0.5939675174013921,
+    0.5912240184757506, 0.5885057471264368, 0.585812356979405, 0.5831435079726651,
+    0.5804988662131519, 0.5778781038374717, 0.5752808988764045, 0.5727069351230425,
+    0.5701559020044543, 0.5676274944567627, 0.565121412803532, 0.5626373626373626,
+    0.5601750547045952, 0.5577342047930284, 0.5553145336225597, 0.5529157667386609,
+    0.5505376344086022, 0.5481798715203426, 0.5458422174840085, 0.5435244161358811,
+    0.5412262156448203, 0.5389473684210526, 0.5366876310272537, 0.534446764091858,
+    0.5322245322245323, 0.5300207039337475, 0.5278350515463918, 0.5256673511293635,
+    0.523517382413088, 0.5213849287169042, 0.5192697768762677, 0.5171717171717172,
+    0.5150905432595574, 0.5130260521042084, 0.5109780439121756, 0.5089463220675944,
+    0.5069306930693069, 0.504930966469428, 0.5029469548133595, 0.5009784735812133,
+};
+FPM_TABLE double FPM_LOG_T[128] = {
+    0.003898640415657309, 0.01165061721997525, 0.019342962843130987, 0.026976587698202083,
+    0.03455238150665973, 0.042071213920687044, 0.049533935122276676, 0.05694137640013845,
+    0.06429435070539725, 0.07159365318700882, 0.078840061707776, 0.08603433734180316,
+    0.09317722485418334, 0.10026945316367517, 0.10731173578908804, 0.11430477128005863,
+    0.12124924363286965, 0.12814582269193006, 0.13499516453750482, 0.1417979118602574,
+    0.1485546943231372, 0.15526612891112396, 0.16193282026931324, 0.16855536102980664,
+    0.17513433212784915, 0.18167030310763463, 0.18816383241818294, 0.19461546769967167,
+    0.2010257460605908, 0.2073951943460706, 0.21372432939771818, 0.22001365830528213,
+    0.2262636786504534, 0.232474878743094, 0.238647737850175, 0.24478272641769092,
+    0.25088030628580943, 0.2569409308975004, 0.26296504550088134, 0.26895308734550394,
+    0.2749054858727992, 0.2808226629008878, 0.2867050328039543, 0.29255300268637746,
+    0.2983669725517973, 0.3041473354672968, 0.3098944777228647, 0.3156087789863033,
+    0.32129061245373425, 0.3269403449958533, 0.3325583373000766, 0.3381449440087164,
+    0.34370051385331846, 0.3492253897852883, 0.354719909102929, 0.3601844035750078,
+    0.3656191995609647, 0.37102461812787263, 0.376400975164253, 0.3817485814908484,
+    0.3870677429684483, 0.3923587606028639, 0.3976219306471385, 0.4028575447010835,
+    0.4080658898082217, 0.41324724855021927, 0.41840189913888387, 0.4235301155058032,
+    0.42863216738969867, 0.4337083204215594, 0.43875883620762796, 0.44378397241030104,
+    0.4487839828270067, 0.4537591174671205, 0.4587096226269767, 0.46363574096303256,
+    0.46853771156323926, 0.4734157700166721, 0.47827014848147026, 0.48310107575113576,
+    0.48790877731923904, 0.4926934754425752, 0.4974553892028189, 0.5021947345667155,
+    0.5069117244448544, 0.5116065687490621, 0.5162794744484545, 0.5209306456241853,
+    0.5255602835229274, 0.5301685866091216, 0.5347557506160276, 0.5393219685956089,
+    0.5438674309672835, 0.5483923255655733, 0.5528968376866776, 0.5573811501340064,
+    0.5618454432626918, 0.5662898950231159, 0.5707146810034716, 0.575119974471388,
+    0.5795059464146423, 0.5838727655809826, 0.588220598517086, 0.5925496096066716,
+    0.5968599611077938, 0.6011518131893347, 0.6054253239667169, 0.6096806495368553,
+    0.6139179440123704, 0.6181373595550788, 0.6223390464087787, 0.6265231529313529,
+    0.6306898256261987, 0.6348392091730102, 0.6389714464579207, 0.6430866786030273,
+    0.6471850449953095, 0.6512666833149582, 0.6553317295631277, 0.6593803180891278,
+    0.6634125816170662, 0.6674286512719563, 0.6714286566053024, 0.6754127256201768,
+    0.6793809847957973, 0.6833335591116206, 0.6872705720709603, 0.691192145724142,
+};
+FPM_TABLE double FPM_EXP2_J[32] = {
+    1.0, 1.0218971486541166, 1.0442737824274138, 1.0671404006768237,
+    1.0905077326652577, 1.1143867425958924, 1.1387886347566916, 1.1637248587775775,
+    1.189207115002721, 1.215247359980469, 1.241857812073484, 1.2690509571917332,
+    1.2968395546510096, 1.3252366431597413, 1.3542555469368927, 1.383909881963832,
+    1.4142135623730951, 1.4451808069770467, 1.4768261459394993, 1.5091644275934228,
+    1.5422108254079407, 1.5759808451078865, 1.6104903319492543, 1.645755478153965,
+    1.681792830507429, 1.718619298122478, 1.7562521603732995, 1.7947090750031072,
+    1.8340080864093424, 1.8741676341103, 1.9152065613971474, 1.9571441241754002,
+};
+
+/* log(x) for a positive, finite, normal binary64 x; ~1e-16 absolute.
+ * x = 2^e m, m in [1, 2); the top 7 bits of m select c_i, r = m/c_i - 1
+ * (|r| <= 2^-8, one rounding by the fused multiply-add) and
+ * log m = T_i + log1p(r), log1p by its Taylor series to degree 7. */
 FPM_FN double fpm_log_d(double x)
 {
     const uint64_t ix = fpm_d2u(x);
-    /* Split x = 2^e * m with m in [sqrt(1/2), sqrt(2)). */
-    const uint64_t t = ix - 0x3fe6a09e667f3bcdull;
-    const int e = (int)(((int64_t)t) >> 52);
-    const double m = fpm_u2d(ix - (t & 0xfff0000000000000ull));
-    const double f = m - 1.0;
-    const double s = f / (2.0 + f);
-    const double z = s * s;
-    /* R = sum_{k=1..8} 2 z^k / (2k+1) */
-    double R = 2.0 / 17.0;
-    R = __builtin_fma(R, z, 2.0 / 15.0);
-    R = __builtin_fma(R, z, 2.0 / 13.0);
-    R = __builtin_fma(R, z, 2.0 / 11.0);
-    R = __builtin_fma(R, z, 2.0 / 9.0);
-    R = __builtin_fma(R, z, 2.0 / 7.0);
-    R = __builtin_fma(R, z, 2.0 / 5.0);
-    R = __builtin_fma(R, z, 2.0 / 3.0);
-    R = R * z;
-    const double hfsq = 0.5 * f * f;
-    const double dk = (double)e;
+    const int e = (int)((ix >> 52) & 0x7ff) - 1023;
+    const double m = fpm_u2d((ix & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    const int i = (int)((ix >> 45) & 127);
+    const double r = __builtin_fma(m, FPM_LOG_INVC[i], -1.0);
+    double q = 1.0 / 7.0;
+    q = __builtin_fma(q, r, -1.0 / 6.0);
+    q = __builtin_fma(q, r, 1.0 / 5.0);
+    q = __builtin_fma(q, r, -0.25);
+    q = __builtin_fma(q, r, 1.0 / 3.0);
+    q = __builtin_fma(q, r, -0.5);
+    const double l1p = __builtin_fma(q, r * r, r);
+    const double de = (double)e;
     /* ln2 = 0.6931471805598903 (exact, 11 trailing zero bits) + 5.4979e-14 */
-    return dk * 0.69314718055989033 -
-           ((hfsq - (s * (hfsq + R) + dk * 5.497923018708371e-14)) - f);
+    return __builtin_fma(de, 0.69314718055989033, FPM_LOG_T[i]) +
+           __builtin_fma(de, 5.497923018708371e-14, l1p);
 }
 
 /* e^t rounded to single precision, for t in [-104, 89]; ~1e-16 relative
- * before the final rounding (overflow and underflow selected at the end). */
+ * before the final rounding (overflow and underflow selected at the end).
+ * t = (32 k' + j) ln2/32 + r, |r| <= ln2/64: e^t = 2^k' 2^(j/32) e^r with
+ * e^r - 1 by its Taylor series to degree 6. */
 FPM_FN float fpm_exp_d_to_f(double t)
 {
     const double tc = __builtin_fmin(__builtin_fmax(t, -104.0), 89.0);
-    const double k = __builtin_rint(tc * 1.4426950408889634);
-    double r = __builtin_fma(k, -0.69314718055989033, tc);
-    r = __builtin_fma(k, -5.497923018708371e-14, r);
-    /* Taylor series of e^r to degree 11 */
-    double p = 1.0 / 39916800.0;
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p = __builtin_fma(p, r, 1.0 / 362880.0);
-    p = __builtin_fma(p, r, 1.0 / 40320.0);
-    p = __builtin_fma(p, r, 1.0 / 5040.0);
-    p = __builtin_fma(p, r, 1.0 / 720.0);
+    const double k = __builtin_rint(tc * 46.16624130844683);
+    /* ln2/32 = 0.021660849392446835 (exact, 16 trailing zero bits) + 5.1456e-14 */
+    double r = __builtin_fma(k, -0.021660849392446835, tc);
+    r = __builtin_fma(k, -5.145609244655338e-14, r);
+    double p = 1.0 / 720.0;
     p = __builtin_fma(p, r, 1.0 / 120.0);
     p = __builtin_fma(p, r, 1.0 / 24.0);
     p = __builtin_fma(p, r, 1.0 / 6.0);
     p = __builtin_fma(p, r, 0.5);
     p = __builtin_fma(p, r * r, r);
-    const double y = 1.0 + p;
     const int ki = (int)k;
-    /* binary64 scaling is exact for ki in [-152, 130] */
-    float res = (float)(y * fpm_u2d((uint64_t)(ki + 1023) << 52));
+    const int j = ki & 31, ke = ki >> 5;
+    const double y = __builtin_fma(FPM_EXP2_J[j], p, FPM_EXP2_J[j]);
+    /* binary64 scaling is exact for ke in [-152, 130] */
+    float res = (float)(y * fpm_u2d((uint64_t)(ke + 1023) << 52));
     res = t > 89.0 ? FPM_INF : res;
     return t < -104.0 ? 0.0f : res;
 }
