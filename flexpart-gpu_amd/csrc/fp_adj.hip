@@ -182,6 +182,7 @@ __global__ void __launch_bounds__(256) adj_step_kernel(AdjArgs g)
     __shared__ float lh[kMaxLevels];
     const StepArgs &a = g.a;
     for (uint32_t k = threadIdx.x; k < a.nz; k += 256) lh[k] = a.level_heights[k];
+    load_math_tab<256>();
     __syncthreads();
     const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
     const Soa s = a.s;

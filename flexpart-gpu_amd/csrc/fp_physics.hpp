@@ -211,21 +211,45 @@ __device__ __forceinline__ Dz tclamp(Dz x, float lo, float hi)
 }
 __device__ __forceinline__ float tpos(float x) { return pos_or_zero(x); }
 __device__ __forceinline__ Dz tpos(Dz x) { return (finite_f(x.v) && x.v > 0.0f) ? x : Dz{0.0f, 0.0f}; }
+// The binary64 log/exp tables of fp_math.h, copied into LDS by the kernels
+// that evaluate the Hanna powers (load_math_tab, before the kernel's first
+// barrier): the same values as the device constants, read at LDS latency
+// rather than through the vector cache that the state and met traffic stream
+// through.  A kernel that calls the lds_* functions must load the copy first.
+constexpr uint32_t kMathTab = 256 + 32;
+__device__ __forceinline__ double *math_tab()
+{
+    __shared__ double t[kMathTab];
+    return t;
+}
+template <uint32_t BLOCK>
+__device__ __forceinline__ void load_math_tab()
+{
+    double *t = math_tab();
+    for (uint32_t k = threadIdx.x; k < kMathTab; k += BLOCK) t[k] = k < 256u ? FPM_LOG_TAB[k] : FPM_EXP2_J[k - 256u];
+}
+__device__ __forceinline__ double lds_log_d(double x) { return fpm_log_d_tab(x, math_tab()); }
+__device__ __forceinline__ float lds_exp_d_to_f(double t) { return fpm_exp_d_to_f_tab(t, math_tab() + 256); }
+__device__ __forceinline__ float lds_powf(float x, float y)
+{
+    return fpm_powf_tab(x, y, math_tab(), math_tab() + 256);
+}
+
 // zeta^y as exp(y log zeta) in binary64 with log zeta = L precomputed
-__device__ __forceinline__ float pow_log(float, float y, double L) { return fpm_exp_d_to_f((double)y * L); }
+__device__ __forceinline__ float pow_log(float, float y, double L) { return lds_exp_d_to_f((double)y * L); }
 __device__ __forceinline__ Dz pow_log(Dz zeta, float y, double L)
 {
-    const float v = fpm_exp_d_to_f((double)y * L);
+    const float v = lds_exp_d_to_f((double)y * L);
     return Dz{v, zeta.v > 0.0f ? v * y / zeta.v * zeta.d : 0.0f};
 }
 // max(zeta, floor)^y with log of the floor Lfloor: exp(y (use ? L : Lfloor))
 __device__ __forceinline__ float pow_log_floor(float, float y, bool use, double L, double Lfloor)
 {
-    return fpm_exp_d_to_f((double)y * (use ? L : Lfloor));
+    return lds_exp_d_to_f((double)y * (use ? L : Lfloor));
 }
 __device__ __forceinline__ Dz pow_log_floor(Dz zeta, float y, bool use, double L, double Lfloor)
 {
-    const float v = fpm_exp_d_to_f((double)y * (use ? L : Lfloor));
+    const float v = lds_exp_d_to_f((double)y * (use ? L : Lfloor));
     return Dz{v, (use && zeta.v > 0.0f) ? v * y / zeta.v * zeta.d : 0.0f};
 }
 
@@ -272,7 +296,7 @@ __device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zet
     const T E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
     T pa = kon<T>(0.0f), pb = kon<T>(0.0f);
     if (!neu) {
-        const double L = fpm_log_d((double)(val(zeta) > 0.0f ? val(zeta) : 1.0f));
+        const double L = lds_log_d((double)(val(zeta) > 0.0f ? val(zeta) : 1.0f));
         pa = val(zeta) > 0.0f ? pow_log(zeta, uns ? 2.0f / 3.0f : 0.8f, L) : kon<T>(0.0f);
         if (uns) pb = pow_log_floor(zeta, -1.0f / 3.0f, val(zeta) >= 1.0e-3f, L, log_1em3);
     }

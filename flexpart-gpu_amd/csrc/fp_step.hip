@@ -158,7 +158,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
                     // below_cloud_scavenging_coefficient_gas (physics/wet_scavenging.rs:213-225)
                     const float pr = pos_or_zero(a.precip[pidx]);
                     lam = 0.0f;
-                    if (pr > 0.0f) lam = pos_or_zero(pos_or_zero(a.wet_a) * fpm_powf(pr, pos_or_zero(a.wet_b)));
+                    if (pr > 0.0f) lam = pos_or_zero(pos_or_zero(a.wet_a) * lds_powf(pr, pos_or_zero(a.wet_b)));
                     frac = a.wet_fraction;
                 } else {
                     lam = a.wet_lambda;
@@ -209,6 +209,7 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 {
     for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
     if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = a.grid.outheights[threadIdx.x];
+    load_math_tab<BLOCK>();
     __syncthreads();
 }
 
