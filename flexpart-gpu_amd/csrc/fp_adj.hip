@@ -176,8 +176,11 @@ __device__ __forceinline__ void scatter_wind_adjoint(const AdjArgs &g, bool vali
 
 // ---------------------------------------------------------------- adjoint step
 
+// Capped at 128 VGPRs (4 wavefronts per SIMD): the uncapped kernel takes 147
+// and runs 3; the cap spills 18 registers to scratch and still runs the
+// reverse sweep 2 % faster.
 template <int NSUB>
-__global__ void __launch_bounds__(256) adj_step_kernel(AdjArgs g)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adj_step_kernel(AdjArgs g)
 {
     __shared__ float lh[kMaxLevels];
     const StepArgs &a = g.a;

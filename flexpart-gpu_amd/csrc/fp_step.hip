@@ -216,10 +216,12 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 // NSUB > 0 specialises the vertical sub-step count at compile time (the
 // production value is 4, timeloop.rs:1111); NSUB < 0 reads it at run time.
 // The kernel is latency-bound at low occupancy; the register budget is
-// capped for 3 wavefronts per SIMD (<= 168 VGPRs), the most that fits
-// without spilling to scratch.
+// capped at 128 VGPRs for 4 wavefronts per SIMD. Every NSUB = 4 variant
+// fits without spilling (the gathered ones need 131 when left uncapped
+// and run 3 wavefronts, 17 % slower); 5 wavefronts (96 VGPRs) spill 17
+// registers to scratch and lose 19 %.
 template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone>
-__global__ void __launch_bounds__(BLOCK, 3) step_kernel(StepArgs a)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) step_kernel(StepArgs a)
 {
     __shared__ float lh[kMaxLevels];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
