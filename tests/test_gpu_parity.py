@@ -113,6 +113,37 @@ def test_step_matches_oracle_bitwise(fp, orc, small, case):
     assert_particles_equal(got, want, f"step {case}")
 
 
+@pytest.mark.parametrize("case", ["production", "three_substeps"])
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+@pytest.mark.parametrize("block", [64, 128, 256, 512, 1024])
+def test_every_launch_shape_matches_oracle_bitwise(fp, orc, small, block, ppt, case):
+    """Each block size the library compiles (each its own register
+    allocation under the 128-VGPR cap), at 1, 2 and 4 particles per thread,
+    with the sub-step count fixed at compile time (production, 4) and read
+    at run time (3): two steps with dry and wet deposition and fused
+    gridding, bit for bit against the oracle."""
+    say(f"step parity: launch {block}x{ppt}, case={case}")
+    kw = dict(STEP_CASES[case], dry_mode=1, dry_vd=2e-3, wet_mode=1, wet_lambda=1e-4, wet_fraction=0.7,
+              grid_fused=True)
+    want = small.particles.copy()
+    oc = om = None
+    with make_ctx(fp, small) as ctx:
+        ctx.set_launch(block, ppt)
+        ctx.configure_grid(small.outheights, small.refine, 0, small.mass_scale)
+        ctx.clear_grid()
+        counter = (3, 0, 0, 0)
+        for _ in range(2):
+            cfg = base_cfg(fp, small, counter=counter, **kw)
+            counter = ctx.step(cfg)
+            orc.step(want, met_dict(small.met), pbl_dict(small.pbl), oracle_params(small, cfg))
+            oc, om = orc.grid_accumulate(want, small.out_shape, small.outheights, small.refine, 0,
+                                         small.mass_scale, oc, om)
+        got = ctx.download_particles()
+        counts, mass = ctx.download_grid()
+    assert_particles_equal(got, want, f"launch {block}x{ppt} {case}")
+    assert np.array_equal(counts, oc) and np.array_equal(mass, om)
+
+
 def test_step_per_slot_forcing_matches_oracle(fp, orc, small):
     say("step parity: per-slot forcing")
     rng = np.random.default_rng(5)
