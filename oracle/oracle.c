@@ -427,6 +427,41 @@ static float update_vertical(float prev, const hanna_t *hn, float dt, float rho,
     return is_finite_f(next) ? next : 0.0f;
 }
 
+/* PBL reflection, langevin_fused.wgsl:481-500: only when h > z_min; up to 10
+ * mirror images about z_min or h, each negating turb_w and cbt, then a clamp
+ * to [z_min, h]. */
+static void pbl_reflect(float *pz, float *tw, int32_t *pcbt, float z_min, float h)
+{
+    if (!(h > z_min)) return;
+    float z = *pz;
+    int32_t cbt = *pcbt;
+    for (int bounce = 0; bounce < 10; ++bounce) {
+        if (z < z_min) {
+            z = 2.0f * z_min - z;
+            cbt = -cbt;
+            *tw = -*tw;
+        } else if (z > h) {
+            z = 2.0f * h - z;
+            cbt = -cbt;
+            *tw = -*tw;
+        } else {
+            break;
+        }
+    }
+    *pz = o_clamp(z, z_min, h);
+    *pcbt = cbt;
+}
+
+void orc_reflect(float *z, float *turb_w, int32_t *cbt, float z_min, float h)
+{
+    pbl_reflect(z, turb_w, cbt, z_min, h);
+}
+
+void orc_hanna_short_w(float ust, float wst, float ol, float h, float z, float out[3])
+{
+    hanna_short_w(ust, wst, ol, h, z, &out[0], &out[1], &out[2]);
+}
+
 /* langevin_fused.wgsl:402-518 (main) */
 static void langevin(orc_particle *p, uint64_t slot, const orc_pbl *pbl,
                      const orc_step_params *sp)
@@ -471,26 +506,7 @@ static void langevin(orc_particle *p, uint64_t slot, const orc_pbl *pbl,
         const float eta = eta_w[sub < 3 ? sub : 3];
         p->turb_w = update_vertical(p->turb_w, &hn, dt_sub, sp->rho_grad_over_rho, eta);
         p->pos_z = p->pos_z + p->turb_w * dt_sub;
-        if (h > z_min) { /* reflection langevin_fused.wgsl:481-500 */
-            float z = p->pos_z;
-            int32_t cbt = p->cbt;
-            for (int bounce = 0; bounce < 10; ++bounce) {
-                if (z < z_min) {
-                    z = 2.0f * z_min - z;
-                    cbt = -cbt;
-                    p->turb_w = -p->turb_w;
-                } else if (z > h) {
-                    z = 2.0f * h - z;
-                    cbt = -cbt;
-                    p->turb_w = -p->turb_w;
-                } else {
-                    break;
-                }
-            }
-            z = o_clamp(z, z_min, h);
-            p->pos_z = z;
-            p->cbt = cbt;
-        }
+        pbl_reflect(&p->pos_z, &p->turb_w, &p->cbt, z_min, h);
         if (sub < sp->n_substeps - 1) {
             float sw, dsw, tw;
             hanna_short_w(hn.ust, hn.wst, hn.ol, hn.h, p->pos_z, &sw, &dsw, &tw);

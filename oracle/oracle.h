@@ -89,6 +89,10 @@ void orc_deposit(orc_particle *p, uint64_t n, const orc_pbl *pbl, const orc_step
 void orc_advect(orc_particle *p, uint64_t n, const orc_met *met, const orc_step_params *sp);
 
 void orc_hanna(float ust, float wst, float ol, float h, float z, float out[13]);
+/* PBL reflection of one particle (langevin_fused.wgsl:481-500), in place */
+void orc_reflect(float *z, float *turb_w, int32_t *cbt, float z_min, float h);
+/* hanna_short_w (langevin_fused.wgsl:296-344): sigma_w, dsigma_w/dz, T_Lw */
+void orc_hanna_short_w(float ust, float wst, float ol, float h, float z, float out[3]);
 
 void orc_grid_accumulate(const orc_particle *p, uint64_t n, const orc_grid_params *gp,
                          uint32_t *counts, uint32_t *mass_fp);

@@ -90,6 +90,9 @@ def lib(variant: str = "fpm") -> C.CDLL:
         L.orc_u32_to_uniform01.restype = C.c_float
         L.orc_box_muller.argtypes = [C.c_float, C.c_float, _f]
         L.orc_hanna.argtypes = [C.c_float] * 5 + [_f]
+        L.orc_hanna_short_w.argtypes = [C.c_float] * 5 + [_f]
+        L.orc_reflect.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                  C.c_float, C.c_float]
         L.orc_update_horizontal.argtypes = [C.c_float] * 5
         L.orc_update_horizontal.restype = C.c_float
         L.orc_update_vertical.argtypes = [C.c_float] * 7
@@ -168,6 +171,21 @@ def hanna(ust, wst, ol, h, z, variant="fpm"):
     lib(variant).orc_hanna(C.c_float(ust), C.c_float(wst), C.c_float(ol), C.c_float(h),
                            C.c_float(z), _fp(out))
     return dict(zip(HANNA_FIELDS, out.tolist()))
+
+
+def hanna_short_w(ust, wst, ol, h, z, variant="fpm"):
+    """hanna_short_w: (sigma_w, dsigma_w/dz, T_Lw) at height z."""
+    out = np.zeros(3, np.float32)
+    lib(variant).orc_hanna_short_w(C.c_float(ust), C.c_float(wst), C.c_float(ol), C.c_float(h),
+                                   C.c_float(z), _fp(out))
+    return dict(zip(("sigw", "dsigwdz", "tlw"), out.tolist()))
+
+
+def reflect(z, turb_w, cbt, z_min, h, variant="fpm"):
+    """PBL reflection of one particle: returns (z, turb_w, cbt)."""
+    zz, tw, cb = C.c_float(z), C.c_float(turb_w), C.c_int32(cbt)
+    lib(variant).orc_reflect(C.byref(zz), C.byref(tw), C.byref(cb), C.c_float(z_min), C.c_float(h))
+    return zz.value, tw.value, cb.value
 
 
 def update_horizontal(prev, sigma, tl, dt, eta, variant="fpm"):
