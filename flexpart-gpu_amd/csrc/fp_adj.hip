@@ -863,7 +863,7 @@ fp::StepArgs step_args(const fp_adj *e, uint32_t k, const fp::Soa &s)
     for (int c = 0; c < 4; ++c) p.counter[c] = e->step_ctr[k][c];
     p.grid_fused = 0;
     fp::fill_step_scalars(a, p);
-    a.ppt = 2;  // block 256 x 2 rows: 3 % faster than one row on the 50M run, the same at 2M
+    a.ppt = 1;  // one particle per thread: the 5-wavefront single-row kernels (10 % faster on the 50M run)
     return a;
 }
 

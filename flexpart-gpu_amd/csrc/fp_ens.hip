@@ -462,7 +462,7 @@ int fp_ens_step(fp_ens *e, uint32_t n_steps, int use_graph, uint32_t *counters_o
     if (use_graph && !e->exec) {
         E_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         const hipError_t h = fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block,
-                                                 e->sub4, e->stream);
+                                                 e->ppt, e->sub4, e->stream);
         hipGraph_t g = nullptr;
         const hipError_t h2 = hipStreamEndCapture(e->stream, &g);
         E_HIP(e, h);
@@ -473,8 +473,8 @@ int fp_ens_step(fp_ens *e, uint32_t n_steps, int use_graph, uint32_t *counters_o
     E_HIP(e, hipEventRecord(e->ev_a, e->stream));
     for (uint32_t k = 0; k < n_steps; ++k) {
         if (use_graph) E_HIP(e, hipGraphLaunch(e->exec, e->stream));
-        else E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->sub4,
-                                          e->stream));
+        else E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt,
+                                          e->sub4, e->stream));
     }
     E_HIP(e, hipEventRecord(e->ev_b, e->stream));
     E_HIP(e, hipEventSynchronize(e->ev_b));

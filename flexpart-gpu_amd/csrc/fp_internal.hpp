@@ -40,6 +40,24 @@ struct Soa {
         return reinterpret_cast<int32_t *>(base + (uint64_t)field * capacity);
     }
     __host__ __device__ uint32_t *u(uint32_t field) const { return base + (uint64_t)field * capacity; }
+    // The address of one field, recomputed from (base, capacity) at each use
+    // with scalar instructions: the empty asm hides the capacity from loop-
+    // invariant hoisting, so a kernel that touches many fields keeps two
+    // scalar register pairs live instead of one pair per field (which spill
+    // to vector-register lanes and cost a v_readlane per access).
+    __host__ __device__ uint32_t *at(uint32_t field) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)capacity);
+        uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(capacity >> 32));
+        asm volatile("" : "+s"(lo), "+s"(hi));
+        return base + (uint64_t)field * (((uint64_t)hi << 32) | lo);
+#else
+        return u(field);
+#endif
+    }
+    __host__ __device__ float *atf(uint32_t field) const { return reinterpret_cast<float *>(at(field)); }
+    __host__ __device__ int32_t *ati(uint32_t field) const { return reinterpret_cast<int32_t *>(at(field)); }
 };
 
 // Concentration-grid configuration shared by the fused and standalone paths.
@@ -258,8 +276,9 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
 hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st);
 // one step of slots [0, a.n) of a.src into the same slots of a.s (stepped words)
 hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st);
+// ppt: particles per thread of every member (a.ppt), selecting the kernel variant
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                           uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st);
+                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, hipStream_t st);
 hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
                            const float *v1, const float *w1, float4 *met, uint32_t npts,
                            hipStream_t st);

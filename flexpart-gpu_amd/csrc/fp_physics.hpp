@@ -28,6 +28,25 @@ __device__ __forceinline__ float pos_or_zero(float v) { return (finite_f(v) && v
 // (a.mul_add(1.0 - t, b * t), physics/interpolation.rs:76-79); u = 1 - t.
 __device__ __forceinline__ float mix(float a, float b, float t, float u) { return __builtin_fmaf(a, u, b * t); }
 
+// Correctly rounded square root.  For x >= 2^-96 this is the compiler's IEEE
+// sequence (hardware square root, then the choice between s - 1 ulp, s and
+// s + 1 ulp by two fused residuals) without the rescaling of arguments below
+// 2^-96 and the class check for zero and infinity, which change nothing in
+// that range; a wavefront with any lane below 2^-96 (or zero, negative or
+// NaN) takes the full sequence.  The step's square roots (Box-Muller radius,
+// Ornstein-Uhlenbeck noise scale, unstable sigma_w) have positive normal
+// arguments, so the short path is the one taken.
+__device__ __forceinline__ float sqrt_cr(float x)
+{
+    if (__builtin_expect(__ballot(!(x >= 0x1p-96f)) != 0ull, 0)) return __builtin_sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r = __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
+}
+
 // ---------------------------------------------------------------- Philox
 
 struct U4 { uint32_t x, y, z, w; };
@@ -62,7 +81,7 @@ __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, fl
 {
     const float ru = clampf(unit24(a), 1.1754944e-38f, 0.99999994f);
     const float au = clampf(unit24(b), 0.0f, 0.99999994f);
-    const float rad = fpm_sqrtf(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1)
+    const float rad = sqrt_cr(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1)
     float sn, cs;
     fpm_sincosf(6.2831855f * au, &sn, &cs);
     g0 = rad * cs;
@@ -114,24 +133,30 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
     const uint32_t ny = a.ny, nz = a.nz;
     const uint32_t r00 = (ax.lo * ny + ay.lo) * nz, r10 = (ax.hi * ny + ay.lo) * nz;
     const uint32_t r01 = (ax.lo * ny + ay.hi) * nz, r11 = (ax.hi * ny + ay.hi) * nz;
-    const uint32_t idx[8] = {r00 + az.lo, r10 + az.lo, r01 + az.lo, r11 + az.lo,
-                             r00 + az.hi, r10 + az.hi, r01 + az.hi, r11 + az.hi};
-    float c[8][6];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float4 p = a.met[2u * idx[k]];
-        const float4 q = a.met[2u * idx[k] + 1u];
-        c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
-        c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
-    }
     const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - alpha;
+    // One z-plane of corners at a time (x, then y within the plane): the
+    // plane values are the x-then-y interpolants of the 8-corner formula, so
+    // the result is the same while only 4 corners are held at once.
+    float pl[2][6];
+#pragma unroll
+    for (int zp = 0; zp < 2; ++zp) {
+        const uint32_t l = zp ? az.hi : az.lo;
+        const uint32_t idx[4] = {r00 + l, r10 + l, r01 + l, r11 + l};
+        float c[4][6];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 p = a.met[2u * idx[k]];
+            const float4 q = a.met[2u * idx[k] + 1u];
+            c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
+            c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            pl[zp][f] = mix(mix(c[0][f], c[1][f], ax.t, ux), mix(c[2][f], c[3][f], ax.t, ux), ay.t, uy);
+    }
     float v[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const float x0 = mix(c[0][f], c[1][f], ax.t, ux), x1 = mix(c[2][f], c[3][f], ax.t, ux);
-        const float x2 = mix(c[4][f], c[5][f], ax.t, ux), x3 = mix(c[6][f], c[7][f], ax.t, ux);
-        v[f] = mix(mix(x0, x1, ay.t, uy), mix(x2, x3, ay.t, uy), az.t, uz);
-    }
+    for (int f = 0; f < 6; ++f) v[f] = mix(pl[0][f], pl[1][f], az.t, uz);
     return Wind{mix(v[0], v[3], alpha, ua), mix(v[1], v[4], alpha, ua), mix(v[2], v[5], alpha, ua)};
 }
 
@@ -190,10 +215,10 @@ __device__ __forceinline__ Dz texp(Dz x)
     const float e = fpm_expf(x.v);
     return Dz{e, e * x.d};
 }
-__device__ __forceinline__ float tsqrt(float x) { return fpm_sqrtf(x); }
+__device__ __forceinline__ float tsqrt(float x) { return sqrt_cr(x); }
 __device__ __forceinline__ Dz tsqrt(Dz x)
 {
-    const float r = fpm_sqrtf(x.v);
+    const float r = sqrt_cr(x.v);
     return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
 }
 __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
@@ -292,8 +317,11 @@ __device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zet
     VRawT<T> v;
     // d0: neutral z/u* (corr), unstable z/L, stable -1.3 u* / h
     v.d0 = (uns || neu ? z : kon<T>(-1.3f * ust)) / (neu ? ust : (uns ? ol : h));
-    // E: neutral e^{-2e-4 corr}, unstable e^{-5 zeta}
-    const T E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
+    // E: neutral e^{-2e-4 corr}, unstable e^{-5 zeta}; d2 below serves the
+    // same two regimes, so a wavefront of stable particles skips both
+    const bool any_nu = __ballot(neu || uns) != 0ull;
+    T E = kon<T>(0.0f);
+    if (any_nu) E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
     T pa = kon<T>(0.0f), pb = kon<T>(0.0f);
     if (!neu) {
         const double L = lds_log_d((double)(val(zeta) > 0.0f ? val(zeta) : 1.0f));
@@ -313,7 +341,8 @@ __device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zet
     // d1: neutral 0.5 z / sigma_w, unstable 0.5 / sigma_w, stable 0.1 h / sigma_w
     const T d1 = (neu ? 0.5f * z : kon<T>(uns ? 0.5f : 0.1f * h)) / sigw;
     // d2: neutral d1 / (1 + 1.5e-3 corr), unstable d1 / h
-    v.d2 = d1 / (neu ? 1.0f + 1.5e-3f * v.d0 : kon<T>(h));
+    v.d2 = kon<T>(0.0f);
+    if (any_nu) v.d2 = d1 / (neu ? 1.0f + 1.5e-3f * v.d0 : kon<T>(h));
     if (neu) {
         v.dsigwdz = -2.0e-4f * sraw;
         v.tlw = v.d2;
@@ -389,7 +418,8 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
         sigv = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
     }
     // unstable and stable: 0.15 h / sigma_u; neutral: T_Lu = T_Lw
-    const T d5 = (0.15f * h) / sigu;
+    T d5 = kon<T>(0.0f);
+    if (__ballot(!t.neu) != 0ull) d5 = (0.15f * h) / sigu;
     if (t.neu) {
         tlu = v.d2;
         tlv = tlu;

@@ -38,7 +38,7 @@ __device__ __forceinline__ bool stepped_field(uint32_t f, uint32_t n_species)
 // step) or at its source slot (kGatherFromSlot: the record is moved there by a
 // compaction after the step) - the slot that selects its Philox counters and
 // per-slot forcing.
-template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone>
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
 __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, float alpha, U4 ctr,
                                            const float *lh, const float *oh, GridLds &gt)
 {
@@ -47,7 +47,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
     // Philox blocks per particle: 2 when the vertical loop takes more than two
     // normals (known at compile time for NSUB >= 0, so both blocks interleave)
     const bool two_blocks = NSUB >= 0 ? NSUB > 2 : a.blocks_per_particle > 1u;
-    for (uint32_t r = 0; r < a.ppt; ++r) {
+    for (uint32_t r = 0; r < (ROWS ? ROWS : a.ppt); ++r) {
         const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
         const bool inb = i < a.n;
         // the slot the state is read from
@@ -63,27 +63,15 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         float px = 0.0f, py = 0.0f, pz = 0.0f, tu = 0.0f, tv = 0.0f, tw = 0.0f;
         float mass[FP_MAX_SPECIES];
         if (inb) {
-            flags = sl.u(F_FLAGS)[g];
-            cx = sl.i(F_CELL_X)[g]; cy = sl.i(F_CELL_Y)[g];
-            px = sl.f(F_POS_X)[g]; py = sl.f(F_POS_Y)[g]; pz = sl.f(F_POS_Z)[g];
-            tu = sl.f(F_TURB_U)[g]; tv = sl.f(F_TURB_V)[g]; tw = sl.f(F_TURB_W)[g];
-            cbt = sl.i(F_CBT)[g];
+            flags = sl.at(F_FLAGS)[g];
+            cx = sl.ati(F_CELL_X)[g]; cy = sl.ati(F_CELL_Y)[g];
+            px = sl.atf(F_POS_X)[g]; py = sl.atf(F_POS_Y)[g]; pz = sl.atf(F_POS_Z)[g];
+            tu = sl.atf(F_TURB_U)[g]; tv = sl.atf(F_TURB_V)[g]; tw = sl.atf(F_TURB_W)[g];
+            cbt = sl.ati(F_CBT)[g];
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) mass[q] = sl.f(F_MASS0 + q)[g];
+                if (q < a.n_species) mass[q] = sl.atf(F_MASS0 + q)[g];
         }
-
-        // ---- Philox blocks per slot (langevin_fused.wgsl:430-451) and the
-        // Box-Muller normals: they depend on the slot index only, so they are
-        // computed while the state loads are in flight.
-        const uint64_t cbase = at * a.blocks_per_particle;
-        const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
-        U4 b1 = U4{0u, 0u, 0u, 0u};
-        if (two_blocks) b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
-        float eu, ev, ew0, ew1, ew2, ew3;
-        gauss_pair(b0.x, b0.y, eu, ev);
-        gauss_pair(b0.z, b0.w, ew0, ew1);
-        gauss_pair(b1.x, b1.y, ew2, ew3);
 
         const bool live = inb && (flags & FLAG_ACTIVE);
         bool grid_valid = false;
@@ -112,7 +100,21 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
             const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
             const uint32_t pidx = pi * a.ny + pj;
-            const Turb t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], pz, a.log_1em3);
+            const float4 pbl0 = a.pbl[2u * pidx], pbl1 = a.pbl[2u * pidx + 1u];
+
+            // ---- Philox blocks per slot (langevin_fused.wgsl:430-451) and the
+            // Box-Muller normals (they depend on the slot index only; drawn here,
+            // while the PBL records load, so that they are not live across the
+            // wind samples)
+            const uint64_t cbase = at * a.blocks_per_particle;
+            const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
+            U4 b1 = U4{0u, 0u, 0u, 0u};
+            if (two_blocks) b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
+            float eu, ev, ew0, ew1, ew2, ew3;
+            gauss_pair(b0.x, b0.y, eu, ev);
+            gauss_pair(b0.z, b0.w, ew0, ew1);
+            gauss_pair(b1.x, b1.y, ew2, ew3);
+            const Turb t = hanna_full(pbl0, pbl1, pz, a.log_1em3);
 
             // ---- horizontal OU over the full step
             tu = ou_horizontal(tu, t.sigu, t.tlu, a.dt, eu);
@@ -178,15 +180,15 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.loss_mode == FP_LOSS_MIN_MASS && mass[0] < a.loss_min_mass) flags &= ~FLAG_ACTIVE;
 
             // ---- store
-            s.i(F_CELL_X)[i] = cx; s.i(F_CELL_Y)[i] = cy;
-            s.f(F_POS_X)[i] = px; s.f(F_POS_Y)[i] = py; s.f(F_POS_Z)[i] = pz;
-            s.f(F_TURB_U)[i] = tu; s.f(F_TURB_V)[i] = tv; s.f(F_TURB_W)[i] = tw;
-            s.f(F_VEL_U)[i] = cu; s.f(F_VEL_V)[i] = cv; s.f(F_VEL_W)[i] = cw;
-            s.i(F_CBT)[i] = cbt;
-            s.u(F_FLAGS)[i] = flags;
+            s.ati(F_CELL_X)[i] = cx; s.ati(F_CELL_Y)[i] = cy;
+            s.atf(F_POS_X)[i] = px; s.atf(F_POS_Y)[i] = py; s.atf(F_POS_Z)[i] = pz;
+            s.atf(F_TURB_U)[i] = tu; s.atf(F_TURB_V)[i] = tv; s.atf(F_TURB_W)[i] = tw;
+            s.atf(F_VEL_U)[i] = cu; s.atf(F_VEL_V)[i] = cv; s.atf(F_VEL_W)[i] = cw;
+            s.ati(F_CBT)[i] = cbt;
+            s.at(F_FLAGS)[i] = flags;
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) s.f(F_MASS0 + q)[i] = mass[q];
+                if (q < a.n_species) s.atf(F_MASS0 + q)[i] = mass[q];
 
             if (a.grid_fused && (flags & FLAG_ACTIVE)) {
                 grid_valid = true;
@@ -198,7 +200,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             // a slot the step leaves unchanged still moves its stepped words
 #pragma unroll
             for (uint32_t f = 0; f < F_COUNT; ++f)
-                if (stepped_field(f, a.n_species)) s.u(f)[i] = sl.u(f)[g];
+                if (stepped_field(f, a.n_species)) s.at(f)[i] = sl.at(f)[g];
         }
         if (a.grid_fused) grid_reduce_wave(gt, a.grid, grid_valid, grid_c, grid_q);
     }
@@ -215,13 +217,13 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 
 // NSUB > 0 specialises the vertical sub-step count at compile time (the
 // production value is 4, timeloop.rs:1111); NSUB < 0 reads it at run time.
-// The kernel is latency-bound at low occupancy; the register budget is
-// capped at 128 VGPRs for 4 wavefronts per SIMD. Every NSUB = 4 variant
-// fits without spilling (the gathered ones need 131 when left uncapped
-// and run 3 wavefronts, 17 % slower); 5 wavefronts (96 VGPRs) spill 17
-// registers to scratch and lose 19 %.
-template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) step_kernel(StepArgs a)
+// ROWS = 1 specialises one particle per thread.  The kernel is bound by
+// vector-ALU issue and the latency of its gathers, so resident wavefronts
+// matter: the single-row kernel needs 84 VGPRs and runs 5 wavefronts per SIMD
+// (6 would spill); with a run-time row loop the compiler keeps loop-invariant
+// values live across the rows and needs 128 VGPRs (4 wavefronts).
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4))) step_kernel(StepArgs a)
 {
     __shared__ float lh[kMaxLevels];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     if (a.grid_fused) grid_lds_clear<BLOCK>(gt);
     load_tables<BLOCK>(a, lh, oh);
     const uint64_t first = (uint64_t)xcd_chunk(blockIdx.x, gridDim.x) * BLOCK * a.ppt;
-    step_chunk<BLOCK, NSUB, GATHER>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
+    step_chunk<BLOCK, NSUB, GATHER, ROWS>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
     if (a.grid_fused) grid_lds_flush<BLOCK>(gt, a.grid);
 }
 
@@ -253,9 +255,9 @@ __global__ void ens_clock_kernel(EnsClock *c)
     c->next_step = k + 1u;
 }
 
-template <uint32_t BLOCK, int NSUB>
-__global__ void __launch_bounds__(BLOCK, 3) ens_step_kernel(const EnsMember *members,
-                                                            const EnsBlock *blocks, const EnsClock *clock)
+template <uint32_t BLOCK, int NSUB, uint32_t ROWS = 0>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4)))
+ens_step_kernel(const EnsMember *members, const EnsBlock *blocks, const EnsClock *clock)
 {
     __shared__ float lh[kMaxLevels];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
@@ -270,7 +272,7 @@ __global__ void __launch_bounds__(BLOCK, 3) ens_step_kernel(const EnsMember *mem
     load_tables<BLOCK>(members[0].a, lh, oh);  // one vertical grid for all members
     const U4 ctr = ctr_add(m.base_ctr[0], m.base_ctr[1], m.base_ctr[2], m.base_ctr[3],
                            (uint64_t)clock->step * m.ctr_per_step);
-    step_chunk<BLOCK, NSUB>(m.a, eb.first, clock->alpha, ctr, lh, oh, gt);
+    step_chunk<BLOCK, NSUB, kGatherNone, ROWS>(m.a, eb.first, clock->alpha, ctr, lh, oh, gt);
     if (m.a.grid_fused) grid_lds_flush<BLOCK>(gt, m.a.grid);
 }
 
@@ -302,8 +304,15 @@ __global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs
 template <uint32_t BLOCK>
 hipError_t launch_block(const StepArgs &a, uint32_t grid, hipStream_t st)
 {
-    if (a.n_substeps == 4u) step_kernel<BLOCK, 4><<<grid, BLOCK, 0, st>>>(a);
-    else step_kernel<BLOCK, -1><<<grid, BLOCK, 0, st>>>(a);
+    // one particle per thread: the single-row variant fits 5 wavefronts per SIMD
+    if (a.ppt == 1u) {
+        if (a.n_substeps == 4u) step_kernel<BLOCK, 4, kGatherNone, 1><<<grid, BLOCK, 0, st>>>(a);
+        else step_kernel<BLOCK, -1, kGatherNone, 1><<<grid, BLOCK, 0, st>>>(a);
+    } else if (a.n_substeps == 4u) {
+        step_kernel<BLOCK, 4><<<grid, BLOCK, 0, st>>>(a);
+    } else {
+        step_kernel<BLOCK, -1><<<grid, BLOCK, 0, st>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -337,8 +346,14 @@ hipError_t launch_gathered(const StepArgs &a, hipStream_t st)
     if ((GATHER == kGatherFromSlot && !a.gather_map) || a.src.base == a.s.base) return hipErrorInvalidValue;
     const uint64_t per_block = 256ull * a.ppt;
     const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
-    if (a.n_substeps == 4u) step_kernel<256, 4, GATHER><<<grid, 256, 0, st>>>(a);
-    else step_kernel<256, -1, GATHER><<<grid, 256, 0, st>>>(a);
+    if (a.ppt == 1u) {
+        if (a.n_substeps == 4u) step_kernel<256, 4, GATHER, 1><<<grid, 256, 0, st>>>(a);
+        else step_kernel<256, -1, GATHER, 1><<<grid, 256, 0, st>>>(a);
+    } else if (a.n_substeps == 4u) {
+        step_kernel<256, 4, GATHER><<<grid, 256, 0, st>>>(a);
+    } else {
+        step_kernel<256, -1, GATHER><<<grid, 256, 0, st>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -360,24 +375,32 @@ hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st)
 
 template <uint32_t BLOCK>
 hipError_t launch_ens_block(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                            uint32_t nblocks, bool sub4, hipStream_t st)
+                            uint32_t nblocks, uint32_t ppt, bool sub4, hipStream_t st)
 {
     ens_clock_kernel<<<1, 64, 0, st>>>(clock);
     if (nblocks == 0) return hipGetLastError();
+    // one particle per thread: the single-row variant fits 5 wavefronts per
+    // SIMD (from 128 threads per block; a 64-thread block would spill)
+    if constexpr (BLOCK >= 128) {
+        if (sub4 && ppt == 1u) {
+            ens_step_kernel<BLOCK, 4, 1><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
+            return hipGetLastError();
+        }
+    }
     if (sub4) ens_step_kernel<BLOCK, 4><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
     else ens_step_kernel<BLOCK, -1><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
     return hipGetLastError();
 }
 
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                           uint32_t nblocks, uint32_t block, bool sub4, hipStream_t st)
+                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, hipStream_t st)
 {
     switch (block) {
-    case 64: return launch_ens_block<64>(members, blocks, clock, nblocks, sub4, st);
-    case 128: return launch_ens_block<128>(members, blocks, clock, nblocks, sub4, st);
-    case 256: return launch_ens_block<256>(members, blocks, clock, nblocks, sub4, st);
-    case 512: return launch_ens_block<512>(members, blocks, clock, nblocks, sub4, st);
-    case 1024: return launch_ens_block<1024>(members, blocks, clock, nblocks, sub4, st);
+    case 64: return launch_ens_block<64>(members, blocks, clock, nblocks, ppt, sub4, st);
+    case 128: return launch_ens_block<128>(members, blocks, clock, nblocks, ppt, sub4, st);
+    case 256: return launch_ens_block<256>(members, blocks, clock, nblocks, ppt, sub4, st);
+    case 512: return launch_ens_block<512>(members, blocks, clock, nblocks, ppt, sub4, st);
+    case 1024: return launch_ens_block<1024>(members, blocks, clock, nblocks, ppt, sub4, st);
     default: return hipErrorInvalidValue;
     }
 }
