@@ -182,9 +182,9 @@ __device__ __forceinline__ void scatter_wind_adjoint(const AdjArgs &g, bool vali
 template <int NSUB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adj_step_kernel(AdjArgs g)
 {
-    __shared__ float lh[kMaxLevels];
+    __shared__ float lh[kLevelTableMax];
     const StepArgs &a = g.a;
-    for (uint32_t k = threadIdx.x; k < a.nz; k += 256) lh[k] = a.level_heights[k];
+    for (uint32_t k = threadIdx.x; k < level_table_words(a.nz); k += 256) lh[k] = a.level_heights[k];
     load_math_tab<256>();
     __syncthreads();
     const uint32_t nsub = NSUB >= 0 ? (uint32_t)NSUB : a.n_substeps;
@@ -950,7 +950,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     C_HIP(hipEventCreate(&e->e1));
     C_HIP(amalloc(&e->met_base, (uint64_t)d.n_met_levels * 3u * npts));
     C_HIP(amalloc(&e->met_packed, (uint64_t)(d.n_met_levels - 1u) * npts * 2u));
-    C_HIP(amalloc(&e->level_heights, (uint64_t)d.met_nz));
+    C_HIP(amalloc(&e->level_heights, (uint64_t)fp::level_table_words(d.met_nz)));
     if (d.n_modes) {
         C_HIP(amalloc(&e->basis_u, (uint64_t)d.n_modes * e->nxy));
         C_HIP(amalloc(&e->basis_v, (uint64_t)d.n_modes * e->nxy));
@@ -1028,7 +1028,7 @@ int fp_adj_met_upload(fp_adj *e, const float *u, const float *v, const float *w,
     const bool metres = lh[nz - 1] > 0.0f;
     if (metres)
         for (uint32_t k = 1; k < nz; ++k) A_CHECK(e, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
-    e->lh_host.assign(lh, lh + nz);
+    fp::level_table(lh, nz, e->lh_host);
     e->has_heights = metres;
     e->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
     e->level_seconds = seconds_between_levels;
@@ -1039,7 +1039,8 @@ int fp_adj_met_upload(fp_adj *e, const float *u, const float *v, const float *w,
             A_HIP(e, hipMemcpyAsync(e->met_base + ((uint64_t)l * 3u + c) * np, f[c], np * 4u, hipMemcpyHostToDevice,
                                     e->stream));
     }
-    A_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), nz * 4u, hipMemcpyHostToDevice, e->stream));
+    A_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->lh_host.size() * 4u, hipMemcpyHostToDevice,
+                            e->stream));
     A_HIP(e, hipStreamSynchronize(e->stream));
     e->met_ready = true;
     e->forward_done = false;

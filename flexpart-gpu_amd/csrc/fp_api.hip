@@ -132,7 +132,7 @@ int validate_levels(fp_ctx *ctx, const float *lh)
     const bool metres = lh[nz - 1] > 0.0f;
     ctx->has_heights = metres;
     ctx->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
-    ctx->level_heights_host.assign(lh, lh + nz);
+    fp::level_table(lh, nz, ctx->level_heights_host);
     return FP_OK;
 }
 
@@ -289,7 +289,7 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
     }
     CREATE_HIP(hipMemsetAsync(ctx->soa[0].base, 0, d.capacity * fp::F_COUNT * 4, ctx->stream));
     CREATE_HIP(dalloc(&ctx->met, npts * 2));
-    CREATE_HIP(dalloc(&ctx->level_heights, (uint64_t)d.met_nz));
+    CREATE_HIP(dalloc(&ctx->level_heights, (uint64_t)fp::level_table_words(d.met_nz)));
     const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
     CREATE_HIP(dalloc(&ctx->pbl, np * 2));
     CREATE_HIP(dalloc(&ctx->precip, np));
@@ -427,7 +427,7 @@ static int upload_met_common(fp_ctx *ctx, const float *const f[6], hipMemcpyKind
         for (int k = 0; k < 6; ++k) src[k] = f[k];
     }
     FP_HIP(ctx, hipMemcpyAsync(ctx->level_heights, ctx->level_heights_host.data(),
-                               ctx->desc.met_nz * 4, hipMemcpyHostToDevice, ctx->stream));
+                               ctx->level_heights_host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     FP_HIP(ctx, fp::launch_pack_met(src[0], src[1], src[2], src[3], src[4], src[5], ctx->met,
                                     (uint32_t)npts, ctx->stream));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -491,8 +491,8 @@ int fp_met_commit_prefetch(fp_ctx *ctx)
     const uint64_t npts = (uint64_t)ctx->desc.met_nx * ctx->desc.met_ny * ctx->desc.met_nz;
     FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->prefetch_done, 0));
     if (int rc = validate_levels(ctx, ctx->prefetch_lh.data())) return rc;
-    FP_HIP(ctx, hipMemcpyAsync(ctx->level_heights, ctx->level_heights_host.data(), ctx->desc.met_nz * 4,
-                               hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(ctx->level_heights, ctx->level_heights_host.data(),
+                               ctx->level_heights_host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     const float *s = ctx->prefetch_stage;
     FP_HIP(ctx, fp::launch_pack_met(s, s + npts, s + 2 * npts, s + 3 * npts, s + 4 * npts, s + 5 * npts, ctx->met,
                                     (uint32_t)npts, ctx->stream));
@@ -1104,3 +1104,24 @@ int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_threa
 }
 
 }  // extern "C"
+
+namespace fp {
+void level_table(const float *lh, uint32_t nz, std::vector<float> &out)
+{
+    out.assign(lh, lh + nz);
+    out.resize(level_table_words(nz), 0.0f);
+    const float top = lh[nz - 1];
+    const float inv_w = top > 0.0f ? (float)kLevelBuckets / top : 0.0f;
+    out[nz] = inv_w;
+    uint16_t first[kLevelBuckets];
+    for (uint32_t b = 0; b < kLevelBuckets; ++b) {
+        // the device maps z to band b = (uint)(z * inv_w) with one rounding of
+        // the product, so every z in band b satisfies z >= b / inv_w (1 - 2^-24)
+        const double zlow = inv_w > 0.0f ? (double)b / (double)inv_w * (1.0 - 1.0e-6) : 0.0;
+        uint32_t k = 1;
+        while (k < nz && (double)lh[k] < zlow) ++k;
+        first[b] = (uint16_t)std::min(k, nz);
+    }
+    std::memcpy(out.data() + nz + 1, first, sizeof first);
+}
+}  // namespace fp

@@ -131,10 +131,11 @@ int met_member_common(fp_ens *e, uint32_t m, const float *const f[6], hipMemcpyK
     bool any = false;
     for (char r : e->met_ready) any = any || r;
     if (any) E_CHECK(e, std::equal(lh, lh + e->nz, e->lh_host.begin()), "members must share the level heights");
-    e->lh_host.assign(lh, lh + e->nz);
+    fp::level_table(lh, e->nz, e->lh_host);
     e->has_heights = metres;
     e->zmax = metres ? lh[e->nz - 1] : (float)(e->nz - 1);
-    E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->nz * 4, hipMemcpyHostToDevice, e->stream));
+    E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->lh_host.size() * 4, hipMemcpyHostToDevice,
+                            e->stream));
     const uint64_t np = e->npts();
     const float *src[6];
     if (kind == hipMemcpyHostToDevice) {
@@ -245,7 +246,7 @@ int fp_ens_create(int device_ordinal, const fp_ens_desc *desc, fp_ens **out)
     C_HIP(dmalloc(&e->soa.base, total * fp::F_COUNT));
     C_HIP(hipMemsetAsync(e->soa.base, 0, total * fp::F_COUNT * 4, e->stream));
     C_HIP(dmalloc(&e->met, npts * 2 * d.n_members));
-    C_HIP(dmalloc(&e->level_heights, (uint64_t)d.met_nz));
+    C_HIP(dmalloc(&e->level_heights, (uint64_t)fp::level_table_words(d.met_nz)));
     const uint64_t np = (uint64_t)d.met_nx * d.met_ny;
     C_HIP(dmalloc(&e->pbl, np * 2));
     C_HIP(dmalloc(&e->precip, np));

@@ -26,6 +26,13 @@ enum Field : uint32_t {
 static_assert(F_COUNT * 4 == sizeof(fp_particle), "particle word count");
 
 constexpr uint32_t kMaxLevels = 256;   // met levels held in LDS by the step kernel
+// Level table (device memory and LDS): the nz level heights, then 1/w as a
+// float, w = top height / kLevelBuckets, then for each height band
+// [b w, (b+1) w) the first level k >= 1 whose height may be at or above a
+// height in the band (uint16, two per word); level_of starts its search there.
+constexpr uint32_t kLevelBuckets = 512;
+constexpr uint32_t kLevelTableMax = kMaxLevels + 1 + kLevelBuckets / 2;
+__host__ __device__ inline uint32_t level_table_words(uint32_t nz) { return nz + 1 + kLevelBuckets / 2; }
 constexpr uint32_t kWave = 64;         // CDNA wavefront width
 constexpr uint32_t kXcds = 8;          // MI355X accelerator complex dies
 
@@ -326,6 +333,10 @@ hipError_t launch_adj_step(const StepArgs &a, float *lam, uint64_t stride, doubl
 // records are gathered from an AoS copy in aos_stage, >= n records).
 hipError_t sort_morton_range(SortWork &w, const fp_sort_params &p, const Soa &in, const Soa &out,
                              fp_particle *aos_stage, uint32_t n, hipStream_t st);
+
+// The level table of nz non-decreasing heights (level-index mode when the top
+// height is not positive), level_table_words(nz) words (fp_api.hip).
+void level_table(const float *lh, uint32_t nz, std::vector<float> &out);
 
 // Shared by the single-run and ensemble entry points (fp_api.hip).
 // Scalar step arguments (timestep, scales, Philox key and counter, sub-steps,

@@ -100,19 +100,21 @@ __device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
     return Axis{l, min(l + 1u, n - 1u), c - fl};
 }
 
-// z [m] -> fractional level.  First k >= 1 with z <= h[k] by bisection over
-// the non-decreasing level table in LDS (same result as the linear scan of
-// advection_dual_wind.wgsl:94-117).
+// z [m] -> fractional level: the first k >= 1 with z <= h[k] (the linear scan
+// of advection_dual_wind.wgsl:94-117) over the non-decreasing level table in
+// LDS.  The search starts at the first level that can qualify for z's height
+// band (the table's band index) and steps up from there: every level below
+// the band's start is lower than z, so the result is that of the full scan.
 __device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz, bool heights)
 {
     if (!heights) return z;
     const float h0 = lh[0];
     if (z <= h0) return h0 > 0.0f ? z / h0 : 0.0f;
-    uint32_t lo = 1, hi = nz;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (z <= lh[mid]) hi = mid; else lo = mid + 1;
-    }
+    const float inv_w = lh[nz];
+    const uint16_t *first = reinterpret_cast<const uint16_t *>(lh + nz + 1);
+    const uint32_t b = (uint32_t)fminf(fmaxf(z * inv_w, 0.0f), (float)(kLevelBuckets - 1u));
+    uint32_t lo = first[b];
+    while (lo < nz && !(z <= lh[lo])) ++lo;
     if (lo >= nz) return (float)(nz - 1);
     const float hp = lh[lo - 1];
     const float den = lh[lo] - hp;

@@ -209,7 +209,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
 template <uint32_t BLOCK>
 __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float *oh)
 {
-    for (uint32_t k = threadIdx.x; k < a.nz; k += BLOCK) lh[k] = a.level_heights[k];
+    for (uint32_t k = threadIdx.x; k < level_table_words(a.nz); k += BLOCK) lh[k] = a.level_heights[k];
     if (threadIdx.x < FP_MAX_OUTPUT_LEVELS) oh[threadIdx.x] = a.grid.outheights[threadIdx.x];
     load_math_tab<BLOCK>();
     __syncthreads();
@@ -225,7 +225,7 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4))) step_kernel(StepArgs a)
 {
-    __shared__ float lh[kMaxLevels];
+    __shared__ float lh[kLevelTableMax];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
     __shared__ GridLds gt;
     if (a.grid_fused) grid_lds_clear<BLOCK>(gt);
@@ -259,7 +259,7 @@ template <uint32_t BLOCK, int NSUB, uint32_t ROWS = 0>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4)))
 ens_step_kernel(const EnsMember *members, const EnsBlock *blocks, const EnsClock *clock)
 {
-    __shared__ float lh[kMaxLevels];
+    __shared__ float lh[kLevelTableMax];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
     __shared__ GridLds gt;
     // Table entries in launch order: the dispatcher deals consecutive blocks to
