@@ -257,11 +257,15 @@ __global__ void ens_clock_kernel(EnsClock *c)
 
 template <uint32_t BLOCK, int NSUB, uint32_t ROWS = 0>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4)))
-ens_step_kernel(const EnsMember *members, const EnsBlock *blocks, const EnsClock *clock)
+ens_step_kernel(const EnsMember *__restrict__ members, const EnsBlock *__restrict__ blocks, const EnsClock *__restrict__ clock)
 {
     __shared__ float lh[kLevelTableMax];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
     __shared__ GridLds gt;
+    // The tables are read through restrict-qualified pointers: nothing the
+    // kernel writes aliases them, so their words are scalar loads held in
+    // scalar registers, as a by-value kernel argument would be (without it
+    // every use re-reads member arguments with vector loads).
     // Table entries in launch order: the dispatcher deals consecutive blocks to
     // the 8 XCDs in turn, so every XCD works on every member.  (Contiguous
     // runs of the table per XCD, as the single-context step uses, give each
