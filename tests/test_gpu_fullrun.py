@@ -24,11 +24,12 @@ def say(msg):
     sys.stdout.flush()
 
 
-def test_24h_run_matches_oracle_bitwise(fp, orc, workloads):
+@pytest.mark.parametrize("ppt", [1, 2])
+def test_24h_run_matches_oracle_bitwise(fp, orc, workloads, ppt):
     import torch
     from test_gpu_parity import oracle_params
 
-    say(f"24 h production run: {N} particles, {STEPS} steps, device vs oracle")
+    say(f"24 h production run: {N} particles, {STEPS} steps, {ppt} particle(s) per thread, device vs oracle")
     w = workloads.global_10m(N)
     dev = torch.device("cuda:0")
     levels = [[torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in lvl] for lvl in w.met.levels]
@@ -46,7 +47,7 @@ def test_24h_run_matches_oracle_bitwise(fp, orc, workloads):
     with fp.GpuTimestep(N, w.met.shape, n_species=1, out_shape=w.out_shape) as ctx:
         ctx.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli)
         ctx.configure_grid(w.outheights, w.refine, 0, w.mass_scale)
-        ctx.set_launch(256, 2)
+        ctx.set_launch(256, ppt)  # the single-row kernel and the row loop
         ctx.upload_store(w.particles)
         bracket, counter = -1, (0, 0, 0, 0)
         for k in range(STEPS):
