@@ -27,6 +27,8 @@ FP_E_INVALID = -1
 FP_E_DEVICE = -2
 FP_E_NOMEM = -3
 FP_E_STATE = -4
+# largest n fp_survivor_map_static accepts (its static scratch: one count per 2048 slots)
+SURVIVOR_MAP_STATIC_MAX = 134_217_728
 
 FLAG_ACTIVE = 1
 MAX_SPECIES = 4
@@ -222,6 +224,7 @@ ABI = {
     "fp_grid_accumulate": (C.c_int, [_vp]),
     "fp_grid_download": (C.c_int, [_vp, _u32, _u32]),
     "fp_compact": (C.c_int, [_vp, C.POINTER(C.c_uint64), _u32]),
+    "fp_survivor_map_static": (C.c_int, [C.c_int, _vp, C.c_uint64, _vp, _vp, _vp]),
     "fp_sort_morton": (C.c_int, [_vp, C.POINTER(SortParams), _u32]),
     "fp_philox_blocks": (C.c_int, [_vp, _u32, _u32, C.c_uint64, _u32]),
     "fp_math_probe": (C.c_int, [_vp, C.c_int, _f, _f, _f, C.c_uint64]),

@@ -23,7 +23,7 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libflexpart_hip.so")
 
-SOURCES = ["fp_step.hip", "fp_aux.hip", "fp_sort.hip", "fp_api.hip", "fp_tune.hip", "fp_ens.hip", "fp_adj.hip", "fp_release.hip", "fp_surface.hip", "fp_output.hip", "fp_met.hip"]
+SOURCES = ["fp_step.hip", "fp_aux.hip", "fp_sort.hip", "fp_api.hip", "fp_tune.hip", "fp_ens.hip", "fp_adj.hip", "fp_release.hip", "fp_surface.hip", "fp_output.hip", "fp_met.hip", "fp_static.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

@@ -22,11 +22,19 @@ static_assert(sizeof(fp_release) == 88, "fp_release layout is part of the ABI");
 
 namespace {
 
-thread_local std::string g_create_error;
+// Message of the last failed call that has no context (fp_ctx_create and the
+// context-free entry points), per thread; a fixed buffer, so that setting it
+// allocates nothing.
+thread_local char g_create_error[512];
+
+void set_create_error(const char *msg)
+{
+    std::snprintf(g_create_error, sizeof g_create_error, "%s", msg);
+}
 
 int fail(fp_ctx *ctx, int code, const std::string &msg)
 {
-    if (ctx) ctx->err = msg; else g_create_error = msg;
+    if (ctx) ctx->err = msg; else set_create_error(msg.c_str());
     return code;
 }
 
@@ -234,7 +242,7 @@ const char *fp_version(void) { return "flexpart-hip 0.1 gfx950"; }
 
 const char *fp_last_error(const fp_ctx *ctx)
 {
-    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+    return ctx ? ctx->err.c_str() : g_create_error;
 }
 
 int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
@@ -270,7 +278,7 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
     ctx->desc = d;
     ctx->dispatch = d.capacity;
     auto bail = [&](int code) {
-        g_create_error = ctx->err;
+        set_create_error(ctx->err.c_str());
         destroy(ctx);
         return code;
     };
@@ -1106,6 +1114,8 @@ int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_threa
 }  // extern "C"
 
 namespace fp {
+void set_context_free_error(const char *msg) { set_create_error(msg); }
+
 void level_table(const float *lh, uint32_t nz, std::vector<float> &out)
 {
     out.assign(lh, lh + nz);

@@ -309,6 +309,28 @@ hipError_t launch_survivor_map(const Soa &in, uint32_t n, uint32_t *tile_counts,
     return hipGetLastError();
 }
 
+namespace {
+__global__ void store_zero_kernel(uint32_t *p) { if (threadIdx.x == 0) *p = 0u; }
+}  // namespace
+
+hipError_t launch_survivor_map_flags(const uint32_t *flags, uint32_t n, uint32_t *tile_counts,
+                                     uint32_t *survivor_map, uint32_t *active_count, hipStream_t st)
+{
+    // a Soa whose every field is the flag array (capacity 0): active_at reads flags[i]
+    Soa in;
+    in.base = const_cast<uint32_t *>(flags);
+    in.capacity = 0;
+    const uint32_t ntiles = (uint32_t)compact_blocks(n);
+    if (ntiles == 0) {
+        store_zero_kernel<<<1, 64, 0, st>>>(active_count);
+        return hipGetLastError();
+    }
+    count_kernel<<<ntiles, kBlock, 0, st>>>(in, n, tile_counts);
+    scan_kernel<<<1, 1024, 0, st>>>(tile_counts, ntiles, active_count);
+    scatter_kernel<false><<<ntiles, kBlock, 0, st>>>(in, in, n, tile_counts, active_count, survivor_map);
+    return hipGetLastError();
+}
+
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
                              hipStream_t st)
 {

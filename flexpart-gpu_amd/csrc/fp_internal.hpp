@@ -300,6 +300,11 @@ hipError_t launch_compact(const Soa &in, const Soa &out, uint32_t n, uint32_t *b
 hipError_t launch_survivor_map(const Soa &in, uint32_t n, uint32_t *block_counts, uint32_t *survivor_map,
                                uint32_t *active_count, hipStream_t st);
 uint64_t compact_blocks(uint32_t n);
+// launch_survivor_map over a plain flag array (bit 0 = active) with the
+// caller's tile-count buffer (compact_blocks(n) words); active_count is
+// written on the device (zero for n = 0, by a kernel: no memset node)
+hipError_t launch_survivor_map_flags(const uint32_t *flags, uint32_t n, uint32_t *tile_counts,
+                                     uint32_t *survivor_map, uint32_t *active_count, hipStream_t st);
 hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2], uint64_t n,
                                 uint32_t *out, hipStream_t st);
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
@@ -333,6 +338,9 @@ hipError_t launch_adj_step(const StepArgs &a, float *lam, uint64_t stride, doubl
 // records are gathered from an AoS copy in aos_stage, >= n records).
 hipError_t sort_morton_range(SortWork &w, const fp_sort_params &p, const Soa &in, const Soa &out,
                              fp_particle *aos_stage, uint32_t n, hipStream_t st);
+
+// Sets the message fp_last_error(NULL) returns (no allocation; fp_api.hip).
+void set_context_free_error(const char *msg);
 
 // The level table of nz non-decreasing heights (level-index mode when the top
 // height is not positive), level_table_words(nz) words (fp_api.hip).

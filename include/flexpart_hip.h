@@ -306,6 +306,21 @@ int fp_grid_download(fp_ctx *ctx, uint32_t *counts, uint32_t *mass_fp);
  * NULL) receives new_slot -> old_slot for the active prefix.  Blocking. */
 int fp_compact(fp_ctx *ctx, uint64_t *active_count, uint32_t *survivor_map);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory - code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc: the survivor map and active count that fp_compact computes,
+ * over a caller-owned device flag array, with no context and no allocation.
+ * flags: n device words (bit 0 = FP_FLAG_ACTIVE); survivor_map: device, room
+ * for n words, receives the active slots in ascending order; active_count:
+ * one device word.  The kernels' scratch (one count per 2048 slots) is a
+ * library-owned 256 KiB static device array per device, so n is at most
+ * FP_SURVIVOR_MAP_STATIC_MAX (FP_E_INVALID beyond).  device_ordinal must be
+ * the current device; stream: a hipStream_t (NULL = default stream).
+ * Asynchronous.  Calls that overlap on one device share the scratch and race. */
+#define FP_SURVIVOR_MAP_STATIC_MAX 134217728ull
+int fp_survivor_map_static(int device_ordinal, const uint32_t *flags, uint64_t n, uint32_t *survivor_map,
+                           uint32_t *active_count, void *stream);
+
 /* ParticleStore::sort_spatially (particles/mod.rs:649-699) on the device:
  * stable sort of the dispatch range by Morton key, inactive slots last.
  * new_to_old (host, may be NULL) receives the permutation.  Blocking only
