@@ -128,7 +128,29 @@ hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *re
         j.out_first = total;
         j.mass = (float)(r.mass_kg / (double)r.particle_count);
         uint64_t idx = released[k];
-        for (; idx < target; ++idx) {
+        // Every sample's grid x depends only on its x stratum and grows (or
+        // falls) monotonically with it in floating point (the stratum centre,
+        // the scaled extent, the offset and the division by dx are each
+        // monotone), and the same holds for y; so when the four extreme
+        // strata corners map inside the domain, every sample does and the
+        // per-sample check below can only stop at the slot limit.
+        bool inside = true;
+        for (uint64_t cx : {(uint64_t)0, j.st.nx - 1u})
+            for (uint64_t cy : {(uint64_t)0, j.st.nx - 1u}) {
+                const Sample p = sample_stratified(r, j.st, cx + cy * j.st.nx);
+                const double gx = (p.lon - g.xlon0) / g.dx, gy = (p.lat - g.ylat0) / g.dy;
+                inside = inside && gx >= 0.0 && gx < (double)(g.nx - 1u) && gy >= 0.0 && gy < (double)(g.ny - 1u);
+            }
+        if (inside) {
+            const uint64_t room = max_particles > total ? max_particles - total : 0u;
+            if (target - idx > room) {
+                idx += room;
+                *status = 2;
+            } else {
+                idx = target;
+            }
+        }
+        for (; *status == 0 && idx < target; ++idx) {
             const Sample p = sample_stratified(r, j.st, idx);
             const double gx = (p.lon - g.xlon0) / g.dx, gy = (p.lat - g.ylat0) / g.dy;
             if (!(gx >= 0.0 && gx < (double)(g.nx - 1u) && gy >= 0.0 && gy < (double)(g.ny - 1u))) {
