@@ -283,6 +283,56 @@ def test_position_and_wind_gradients_match_central_differences(fp, smooth_case, 
     assert not bad, bad
 
 
+def test_smooth_gradients_match_binary64_central_differences(fp, smooth_case, smooth_receptors):
+    """The adjoint against central differences of the binary64 restatement of
+    the same run (oracle/forward64.py, pinned against the float32 oracle):
+    rounding noise no longer limits the step, so steps of 1e-4 cells, 1e-3 m
+    and 1e-5 m/s resolve the derivative of the smooth configuration, and
+    positions, wind coefficients and emissions must agree within 1e-3 of the
+    value (entries below 1 % of their group's largest gradient: within 1e-5
+    of that scale)."""
+    import forward64 as F
+    c = smooth_case
+    nodes, obs = smooth_receptors
+    say("adjoint vs binary64 central differences: positions, wind coefficients, emissions")
+    with make_adj(fp, c) as adj:
+        adj.set_receptors(nodes, obs)
+        cfg = cfg_of(fp, c)
+        adj.forward(cfg, c.emission, c.position, c.coeff)
+        gq, gr, gc = adj.backward()
+
+    def loss(q, r, k):
+        return F.adjoint_forward(c, q, r, k, nodes, obs, 6, 4, OBS_NZ, OBS_DZ, tuple(cfg.counter),
+                                 dry_vd=cfg.dry_vd)[1]
+
+    q0, r0, k0 = (c.emission.astype(np.float64), c.position.astype(np.float64), c.coeff.astype(np.float64))
+
+    def fd(which, idx, h):
+        out = []
+        for sgn in (1.0, -1.0):
+            q, r, k = q0.copy(), r0.copy(), k0.copy()
+            {"q": q, "r": r, "c": k}[which][idx] += sgn * h
+            out.append(loss(q, r, k))
+        return (out[0] - out[1]) / (2.0 * h)
+
+    rows = []
+    for s in (0, 1, 3, 5):
+        for d, h in ((0, 1e-4), (1, 1e-4), (2, 1e-3)):
+            rows.append(("r", (s, d), float(gr[s, d]), fd("r", (s, d), h)))
+    for k in range(len(COEFF)):
+        rows.append(("c", k, float(gc[k]), fd("c", k, 1e-5)))
+    for s in range(len(COUNTS)):
+        rows.append(("q", s, float(gq[s]), fd("q", s, 1e-4 * float(q0[s]))))
+    scale = {w: max(abs(f) for ww, _, _, f in rows if ww == w) for w in ("r", "c", "q")}
+    bad = []
+    for w, i, g, f in rows:
+        tol = 1e-3 * abs(f) if abs(f) >= 1e-2 * scale[w] else 1e-5 * scale[w]
+        say(f"  {w}{i}: adjoint {g: .8e}  fd64 {f: .8e}  rel {abs(g - f) / max(abs(f), 1e-300):.2e}")
+        if abs(g - f) > tol:
+            bad.append((w, i, g, f))
+    assert not bad, bad
+
+
 def test_autograd_function_returns_the_backward_gradients(fp, case, receptors):
     import torch
     c = case
