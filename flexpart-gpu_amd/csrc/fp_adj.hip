@@ -906,7 +906,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
         if (d.particles_per_source[k] == 0) return afail(nullptr, FP_E_INVALID, "every source needs >= 1 particle");
         total += d.particles_per_source[k];
     }
-    if (total > 0x7fffffffull) return afail(nullptr, FP_E_INVALID, "total particle count must be < 2^31");
+    if (total > fp::kMaxSlots) return afail(nullptr, FP_E_INVALID, "total particle count must be <= 2^30");
     const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
     if (npts * 2 >= 0x80000000ull) return afail(nullptr, FP_E_INVALID, "met grid too large");
     int ndev = 0;

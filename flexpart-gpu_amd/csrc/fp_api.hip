@@ -250,8 +250,8 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
     if (!out || !desc) return fail(nullptr, FP_E_INVALID, "null argument");
     *out = nullptr;
     const fp_ctx_desc &d = *desc;
-    if (d.capacity == 0 || d.capacity > 0x7fffffffull)
-        return fail(nullptr, FP_E_INVALID, "capacity must be in [1, 2^31)");
+    if (d.capacity == 0 || d.capacity > fp::kMaxSlots)
+        return fail(nullptr, FP_E_INVALID, "capacity must be in [1, 2^30]");
     if (d.met_nx == 0 || d.met_ny == 0 || d.met_nz == 0)
         return fail(nullptr, FP_E_INVALID, "met grid dimensions must be non-zero");
     if (d.met_nz > fp::kMaxLevels) return fail(nullptr, FP_E_INVALID, "met_nz exceeds 256");

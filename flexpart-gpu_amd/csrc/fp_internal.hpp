@@ -26,6 +26,9 @@ enum Field : uint32_t {
 static_assert(F_COUNT * 4 == sizeof(fp_particle), "particle word count");
 
 constexpr uint32_t kMaxLevels = 256;   // met levels held in LDS by the step kernel
+// Particle slots per SoA set: the step kernels address a field word by a
+// 32-bit byte offset from the field's base (slot << 2).
+constexpr uint64_t kMaxSlots = 1ull << 30;
 // Level table (device memory and LDS): the nz level heights, then 1/w as a
 // float, w = top height / kLevelBuckets, then for each height band
 // [b w, (b+1) w) the first level k >= 1 whose height may be at or above a
@@ -65,6 +68,24 @@ struct Soa {
     }
     __host__ __device__ float *atf(uint32_t field) const { return reinterpret_cast<float *>(at(field)); }
     __host__ __device__ int32_t *ati(uint32_t field) const { return reinterpret_cast<int32_t *>(at(field)); }
+    // Word `slot` of a field addressed as the field's scalar base plus a
+    // 32-bit byte offset (slot < 2^30), the form of a global access with a
+    // scalar base register and a vector offset: no 64-bit vector address
+    // arithmetic per access.
+#if defined(__HIP_DEVICE_COMPILE__)
+    template <class T> __device__ __attribute__((address_space(1))) T &w(uint32_t field, uint32_t slot) const
+    {
+        uint64_t p = reinterpret_cast<uint64_t>(at(field));
+        asm volatile("" : "+s"(p));
+        using G = __attribute__((address_space(1))) char *;
+        return *reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + (slot << 2));
+    }
+#else
+    template <class T> __host__ __device__ T &w(uint32_t field, uint32_t slot) const
+    {
+        return reinterpret_cast<T *>(u(field))[slot];
+    }
+#endif
 };
 
 // Concentration-grid configuration shared by the fused and standalone paths.

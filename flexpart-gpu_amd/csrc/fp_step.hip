@@ -63,14 +63,14 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         float px = 0.0f, py = 0.0f, pz = 0.0f, tu = 0.0f, tv = 0.0f, tw = 0.0f;
         float mass[FP_MAX_SPECIES];
         if (inb) {
-            flags = sl.at(F_FLAGS)[g];
-            cx = sl.ati(F_CELL_X)[g]; cy = sl.ati(F_CELL_Y)[g];
-            px = sl.atf(F_POS_X)[g]; py = sl.atf(F_POS_Y)[g]; pz = sl.atf(F_POS_Z)[g];
-            tu = sl.atf(F_TURB_U)[g]; tv = sl.atf(F_TURB_V)[g]; tw = sl.atf(F_TURB_W)[g];
-            cbt = sl.ati(F_CBT)[g];
+            flags = sl.w<uint32_t>(F_FLAGS, (uint32_t)g);
+            cx = sl.w<int32_t>(F_CELL_X, (uint32_t)g); cy = sl.w<int32_t>(F_CELL_Y, (uint32_t)g);
+            px = sl.w<float>(F_POS_X, (uint32_t)g); py = sl.w<float>(F_POS_Y, (uint32_t)g); pz = sl.w<float>(F_POS_Z, (uint32_t)g);
+            tu = sl.w<float>(F_TURB_U, (uint32_t)g); tv = sl.w<float>(F_TURB_V, (uint32_t)g); tw = sl.w<float>(F_TURB_W, (uint32_t)g);
+            cbt = sl.w<int32_t>(F_CBT, (uint32_t)g);
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) mass[q] = sl.atf(F_MASS0 + q)[g];
+                if (q < a.n_species) mass[q] = sl.w<float>(F_MASS0 + q, (uint32_t)g);
         }
 
         const bool live = inb && (flags & FLAG_ACTIVE);
@@ -180,15 +180,15 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.loss_mode == FP_LOSS_MIN_MASS && mass[0] < a.loss_min_mass) flags &= ~FLAG_ACTIVE;
 
             // ---- store
-            s.ati(F_CELL_X)[i] = cx; s.ati(F_CELL_Y)[i] = cy;
-            s.atf(F_POS_X)[i] = px; s.atf(F_POS_Y)[i] = py; s.atf(F_POS_Z)[i] = pz;
-            s.atf(F_TURB_U)[i] = tu; s.atf(F_TURB_V)[i] = tv; s.atf(F_TURB_W)[i] = tw;
-            s.atf(F_VEL_U)[i] = cu; s.atf(F_VEL_V)[i] = cv; s.atf(F_VEL_W)[i] = cw;
-            s.ati(F_CBT)[i] = cbt;
-            s.at(F_FLAGS)[i] = flags;
+            s.w<int32_t>(F_CELL_X, (uint32_t)i) = cx; s.w<int32_t>(F_CELL_Y, (uint32_t)i) = cy;
+            s.w<float>(F_POS_X, (uint32_t)i) = px; s.w<float>(F_POS_Y, (uint32_t)i) = py; s.w<float>(F_POS_Z, (uint32_t)i) = pz;
+            s.w<float>(F_TURB_U, (uint32_t)i) = tu; s.w<float>(F_TURB_V, (uint32_t)i) = tv; s.w<float>(F_TURB_W, (uint32_t)i) = tw;
+            s.w<float>(F_VEL_U, (uint32_t)i) = cu; s.w<float>(F_VEL_V, (uint32_t)i) = cv; s.w<float>(F_VEL_W, (uint32_t)i) = cw;
+            s.w<int32_t>(F_CBT, (uint32_t)i) = cbt;
+            s.w<uint32_t>(F_FLAGS, (uint32_t)i) = flags;
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) s.atf(F_MASS0 + q)[i] = mass[q];
+                if (q < a.n_species) s.w<float>(F_MASS0 + q, (uint32_t)i) = mass[q];
 
             if (a.grid_fused && (flags & FLAG_ACTIVE)) {
                 grid_valid = true;
@@ -200,7 +200,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             // a slot the step leaves unchanged still moves its stepped words
 #pragma unroll
             for (uint32_t f = 0; f < F_COUNT; ++f)
-                if (stepped_field(f, a.n_species)) s.at(f)[i] = sl.at(f)[g];
+                if (stepped_field(f, a.n_species)) s.w<uint32_t>(f, (uint32_t)i) = sl.w<uint32_t>(f, (uint32_t)g);
         }
         if (a.grid_fused) grid_reduce_wave(gt, a.grid, grid_valid, grid_c, grid_q);
     }
