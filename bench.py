@@ -582,11 +582,11 @@ def bench_ensemble(args) -> int:
 
 
 def bench_adjoint(args) -> int:
-    print(json.dumps(adjoint_result(args.steps)), flush=True)
+    print(json.dumps(adjoint_result(args.steps, jump_stats=True)), flush=True)
     return 0
 
 
-def adjoint_result(n_steps: int) -> dict:
+def adjoint_result(n_steps: int, jump_stats: bool = False) -> dict:
     """The differentiable run: 2M particles from 1024 sources (lognormal
     emissions, uniform-random positions, heights U[10, 500] m), 12 h (72 steps
     of 600 s) on the 360 x 181 x 60 global met with 3 time levels, 64 wind
@@ -641,13 +641,31 @@ def adjoint_result(n_steps: int) -> dict:
         bwd.append(b)
     fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
 
+    offs = np.concatenate([[0], np.cumsum(c.counts.astype(np.int64))])
+    jumps = {}
+
     def fd(which, idx, h):
-        vals = []
+        vals, states = [], []
         for sgn in (1.0, -1.0):
             q, r, k = c.emission.copy(), c.position.copy(), c.coeff.copy()
             arr = {"q": q, "r": r, "c": k}[which]
             arr[idx] = np.float32(arr[idx] + sgn * h)
             vals.append(adj.forward(cfg, q, r, k)[1])
+            if jump_stats and which != "q":
+                states.append(adj.download_state())
+        if states:
+            # particles whose trajectories under +h and -h differ by far more
+            # than the smooth response: they crossed a PBL cell or regime
+            # boundary, a reflection or an Ornstein-Uhlenbeck branch switch
+            # differently (the jumps the pathwise derivative leaves out)
+            a, b = states
+            dz = np.abs(a["pos_z"].astype(np.float64) - b["pos_z"])
+            dx = np.abs((a["cell_x"] - b["cell_x"]) + (a["pos_x"].astype(np.float64) - b["pos_x"]))
+            jumped = (dz > 1.0) | (dx > 0.01)
+            sl = slice(offs[idx[0]], offs[idx[0] + 1]) if which == "r" else slice(0, len(dz))
+            key = [int(v) for v in idx] if isinstance(idx, tuple) else [int(idx)]
+            jumps[f"{which}{key}"] = {
+                "particles": int(sl.stop - sl.start), "share_jumped": float(jumped[sl].mean())}
         base = {"q": c.emission, "r": c.position, "c": c.coeff}[which][idx]
         return (vals[0] - vals[1]) / float(np.float32(base + h) - np.float32(base - h))
 
@@ -691,6 +709,7 @@ def adjoint_result(n_steps: int) -> dict:
             "forward_particle_steps_per_s": particle_steps / (fwd_ms / 1e3),
             "loss": loss,
             "fd_checks": checks,
+            "fd_jumps": jumps or None,
         },
         "roofline": None,
         "cpu_baseline": None,
