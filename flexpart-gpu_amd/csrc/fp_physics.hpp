@@ -51,6 +51,13 @@ __device__ __forceinline__ float sqrt_cr(float x)
 
 struct U4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ c in one VALU: gfx950's three-input bit operation with truth table
+// 0x96 (the parity of its three inputs); CDNA has no v_xor3_b32.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -58,7 +65,8 @@ __device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1)
         // one 32x32->64 multiply per lane and word (v_mad_u64_u32)
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+        c = U4{xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1, xor3((uint32_t)(p0 >> 32), c.w, k1),
+               (uint32_t)p0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
