@@ -54,8 +54,25 @@ struct fp_ens {
 
     uint32_t block = 256, ppt = 1;
     fp::SortWork sortw;  // allocated by the first fp_ens_sort_morton
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // captured step sequences: [0] one step, [1] `multi_steps` consecutive
+    // steps (an output interval replayed as one graph launch)
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    uint32_t multi_steps = 0;
+    // the launch a graph was captured with: table addresses and launch
+    // shape (the tables' contents are read at run time, so a re-configured
+    // ensemble with the same signature replays its graphs unchanged)
+    struct LaunchSig {
+        const void *members, *blocks, *clock;
+        uint32_t nblocks, block, ppt;
+        bool sub4;
+        bool operator==(const LaunchSig &o) const
+        {
+            return members == o.members && blocks == o.blocks && clock == o.clock && nblocks == o.nblocks &&
+                   block == o.block && ppt == o.ppt && sub4 == o.sub4;
+        }
+    } graph_sig[2] = {};
+    LaunchSig sig() const { return LaunchSig{d_members, d_blocks, d_clock, nblocks, block, ppt, sub4}; }
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     double last_ms = 0.0;
 
@@ -97,12 +114,44 @@ hipError_t dmalloc(T **p, uint64_t count)
     return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
 }
 
+void drop_graph_slot(fp_ens *e, int s)
+{
+    if (e->exec[s]) (void)hipGraphExecDestroy(e->exec[s]);
+    if (e->graph[s]) (void)hipGraphDestroy(e->graph[s]);
+    e->exec[s] = nullptr;
+    e->graph[s] = nullptr;
+    if (s == 1) e->multi_steps = 0;
+}
+
 void drop_graph(fp_ens *e)
 {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
+    drop_graph_slot(e, 0);
+    drop_graph_slot(e, 1);
+}
+
+// Capture n_steps consecutive sync steps (clock + step launch each) into
+// graph slot s.  The clock lives on the device, so the replays need no host
+// arguments.
+int capture_steps(fp_ens *e, int s, uint32_t n_steps)
+{
+    drop_graph_slot(e, s);
+    E_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t h = hipSuccess;
+    for (uint32_t k = 0; k < n_steps && h == hipSuccess; ++k)
+        h = fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt, e->sub4,
+                                e->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t h2 = hipStreamEndCapture(e->stream, &g);
+    if (h != hipSuccess || h2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        E_HIP(e, h);
+        E_HIP(e, h2);
+    }
+    e->graph[s] = g;
+    E_HIP(e, hipGraphInstantiate(&e->exec[s], e->graph[s], nullptr, nullptr, 0));
+    e->graph_sig[s] = e->sig();
+    if (s == 1) e->multi_steps = n_steps;
+    return FP_OK;
 }
 
 void destroy(fp_ens *e)
@@ -159,7 +208,6 @@ int met_member_common(fp_ens *e, uint32_t m, const float *const f[6], hipMemcpyK
 // configured members (the clock is left as it is).
 int rebuild_tables(fp_ens *e)
 {
-    drop_graph(e);
     std::vector<fp::EnsBlock> blocks;
     const uint64_t per_block = (uint64_t)e->block * e->ppt;
     for (uint32_t m = 0; m < e->n_members; ++m) {
@@ -389,7 +437,6 @@ int fp_ens_set_launch(fp_ens *e, uint32_t block, uint32_t ppt)
     E_CHECK(e, ppt >= 1 && ppt <= 16, "particles_per_thread must be 1..16");
     e->block = block;
     e->ppt = ppt;
-    drop_graph(e);
     // a configured ensemble keeps its configuration and clock
     if (e->configured) return rebuild_tables(e);
     return FP_OK;
@@ -410,7 +457,6 @@ int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, c
     E_CHECK(e, p->loss_mode <= FP_LOSS_MIN_MASS, "unknown loss_mode");
     E_STATE(e, !p->grid_fused || e->grid_configured, "fused gridding before fp_ens_grid_configure");
     E_CHECK(e, bracket_s > 0.0 && std::isfinite(t0_in_bracket), "invalid bracket time");
-    drop_graph(e);
     e->members.assign(e->n_members, fp::EnsMember{});
     const uint64_t np = e->npts();
     for (uint32_t m = 0; m < e->n_members; ++m) {
@@ -460,22 +506,28 @@ int fp_ens_step(fp_ens *e, uint32_t n_steps, int use_graph, uint32_t *counters_o
     E_CHECK(e, e, "null ensemble");
     E_STATE(e, e->configured, "fp_ens_step before fp_ens_configure");
     if (n_steps == 0) return FP_OK;
-    if (use_graph && !e->exec) {
-        E_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t h = fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block,
-                                                 e->ppt, e->sub4, e->stream);
-        hipGraph_t g = nullptr;
-        const hipError_t h2 = hipStreamEndCapture(e->stream, &g);
-        E_HIP(e, h);
-        E_HIP(e, h2);
-        e->graph = g;
-        E_HIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    // graph path: the call's steps as replays of one captured sequence of
+    // min(n_steps, kMaxGraphSteps) steps, kept for later calls of that
+    // length, and the remainder from the one-step graph
+    constexpr uint32_t kMaxGraphSteps = 36;
+    const uint32_t group = std::min(n_steps, kMaxGraphSteps);
+    const uint32_t n_groups = group > 1 ? n_steps / group : 0;
+    const uint32_t n_single = n_steps - n_groups * group;
+    if (use_graph) {
+        const auto now = e->sig();
+        if (n_groups && !(e->exec[1] && e->multi_steps == group && e->graph_sig[1] == now))
+            if (int rc = capture_steps(e, 1, group)) return rc;
+        if (n_single && !(e->exec[0] && e->graph_sig[0] == now))
+            if (int rc = capture_steps(e, 0, 1)) return rc;
     }
     E_HIP(e, hipEventRecord(e->ev_a, e->stream));
-    for (uint32_t k = 0; k < n_steps; ++k) {
-        if (use_graph) E_HIP(e, hipGraphLaunch(e->exec, e->stream));
-        else E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt,
-                                          e->sub4, e->stream));
+    if (use_graph) {
+        for (uint32_t k = 0; k < n_groups; ++k) E_HIP(e, hipGraphLaunch(e->exec[1], e->stream));
+        for (uint32_t k = 0; k < n_single; ++k) E_HIP(e, hipGraphLaunch(e->exec[0], e->stream));
+    } else {
+        for (uint32_t k = 0; k < n_steps; ++k)
+            E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt,
+                                         e->sub4, e->stream));
     }
     E_HIP(e, hipEventRecord(e->ev_b, e->stream));
     E_HIP(e, hipEventSynchronize(e->ev_b));

@@ -540,16 +540,22 @@ int fp_ens_grid_download_member(fp_ens *ens, uint32_t member, uint32_t *counts, 
  * is (time_in_bracket + k * direction * dt) / bracket_seconds, clamped to
  * [0, 1], at clock step k (k = 0 at the first step after this call).
  * dispatch: per-member slot counts (NULL = capacities).  Rebuilds the
- * launch table and drops a captured graph. */
+ * launch table; captured graphs are kept while the table's size and the
+ * launch shape stay the same (they read the tables at run time). */
 int fp_ens_configure(fp_ens *ens, const fp_step_params *base, const uint32_t *keys,
                      const uint32_t *counters, const uint64_t *dispatch, double time_in_bracket_s,
                      double bracket_s);
-/* n_steps steps of every member; use_graph = 1 captures the step once
- * (hipGraph) and replays it.  counters_out (4 per member, may be NULL)
- * receives each member's counter after the last step. */
+/* n_steps steps of every member; use_graph = 1 replays captured hipGraphs:
+ * min(n_steps, 36) consecutive steps (an output interval, say) captured as
+ * one graph that later calls of the same length replay again, the rest of
+ * the call from a one-step graph.  A graph is captured again when the
+ * launch table's size or the launch shape changed since its capture.
+ * counters_out (4 per member, may be NULL) receives each member's counter
+ * after the last step. */
 int fp_ens_step(fp_ens *ens, uint32_t n_steps, int use_graph, uint32_t *counters_out);
 /* Launch shape of the step; a configured ensemble keeps its configuration
- * and step clock (the launch tables are rebuilt, a captured graph dropped). */
+ * and step clock (the launch tables are rebuilt, graphs captured again at
+ * the next fp_ens_step). */
 int fp_ens_set_launch(fp_ens *ens, uint32_t block_size, uint32_t particles_per_thread);
 /* Morton re-sort of every member's dispatch range: each member ends in the
  * order fp_sort_morton gives for it alone (replaces ParticleStore::

@@ -163,6 +163,39 @@ def test_graph_replays_many_steps_equal_direct_launches(fp, small):
         assert np.array_equal(ga[m][0], gb[m][0]) and np.array_equal(ga[m][1], gb[m][1])
 
 
+def test_multi_step_graphs_equal_direct_launches(fp, small):
+    """Multi-step graphs (a 36-step group plus one-step remainders, a 6-step
+    output interval replayed twice) kept across re-configurations with the
+    same launch table and captured again after a launch-shape or dispatch
+    change, against the same calls with direct launches."""
+    from test_gpu_parity import base_cfg
+    say("ensemble: multi-step graphs vs direct launches over re-configurations")
+    keys = [member_key(small, m) for m in range(len(CAPS))]
+    outs = []
+    for use_graph in (False, True):
+        with build_ensemble(fp, small) as ens:
+            cfg = base_cfg(fp, small, **CASES["production"])
+            ens.configure(cfg, keys=keys, counters=[member_counter(m) for m in range(len(CAPS))],
+                          dispatch=DISPATCH, time_in_bracket_s=T0, bracket_s=BRACKET)
+            ctr = ens.step(40, use_graph=use_graph)          # 36-step graph + 4 one-step replays
+            ens.configure(cfg, keys=keys, counters=ctr, dispatch=DISPATCH, time_in_bracket_s=T0,
+                          bracket_s=BRACKET)                  # same table: graphs kept
+            ens.step(6, use_graph=use_graph)
+            ctr = ens.step(6, use_graph=use_graph)
+            ens.set_launch(128, 2)                            # new shape: captured again
+            ctr = ens.step(3, use_graph=use_graph)
+            ens.configure(cfg, keys=keys, counters=ctr, dispatch=CAPS, time_in_bracket_s=T0 + 600.0,
+                          bracket_s=BRACKET)                  # new dispatch: captured again
+            ctr = ens.step(3, use_graph=use_graph)
+            outs.append(([ens.download_member(m) for m in range(len(CAPS))], ctr,
+                         [ens.download_grid_member(m) for m in range(len(CAPS))]))
+    (pa, ca, ga), (pb, cb, gb) = outs
+    assert np.array_equal(ca, cb)
+    for m in range(len(CAPS)):
+        assert_particles_equal(pa[m], pb[m], f"member {m} multi-step graphs vs direct")
+        assert np.array_equal(ga[m][0], gb[m][0]) and np.array_equal(ga[m][1], gb[m][1])
+
+
 def test_member_equals_single_run_context(fp, small):
     """A member advanced in an ensemble equals the same member stepped alone
     by fp_step with the clock's alpha."""
