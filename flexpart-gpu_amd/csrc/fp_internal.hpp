@@ -77,8 +77,10 @@ struct Soa {
     {
         uint64_t p = reinterpret_cast<uint64_t>(at(field));
         asm volatile("" : "+s"(p));
+        uint32_t off = slot << 2;
+        asm("" : "+v"(off));
         using G = __attribute__((address_space(1))) char *;
-        return *reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + (slot << 2));
+        return *reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + off);
     }
 #else
     template <class T> __host__ __device__ T &w(uint32_t field, uint32_t slot) const
