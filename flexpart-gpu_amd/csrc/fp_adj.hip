@@ -104,8 +104,8 @@ __device__ __forceinline__ WindGrad sample_wind_grad(const StepArgs &a, const fl
     for (int k = 0; k < 8; ++k) {
         const uint32_t ix = (k & 1) ? ax.hi : ax.lo, iy = (k & 2) ? ay.hi : ay.lo, iz = (k & 4) ? az.hi : az.lo;
         const uint32_t idx = (ix * ny + iy) * nz + iz;
-        const float4 p = a.met[2u * idx];
-        const float4 q = a.met[2u * idx + 1u];
+        const float4 p = load_at(a.met, idx << 5);
+        const float4 q = load_at(a.met, idx << 5, 16u);
         const float wx = (k & 1) ? ax.t : ux, wy = (k & 2) ? ay.t : uy, wz = (k & 4) ? az.t : uz;
         const float sx = (k & 1) ? 1.0f : -1.0f, sy = (k & 2) ? 1.0f : -1.0f, sz = (k & 4) ? 1.0f : -1.0f;
         // time-blended corner value of u, v, w
@@ -908,7 +908,7 @@ int fp_adj_create(int device_ordinal, const fp_adj_desc *desc, fp_adj **out)
     }
     if (total > fp::kMaxSlots) return afail(nullptr, FP_E_INVALID, "total particle count must be <= 2^30");
     const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
-    if (npts * 2 >= 0x80000000ull) return afail(nullptr, FP_E_INVALID, "met grid too large");
+    if (npts > fp::kMaxMetPoints) return afail(nullptr, FP_E_INVALID, "met grid too large (at most 2^27 points)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return afail(nullptr, FP_E_DEVICE, "no HIP device");
     if (device_ordinal < 0 || device_ordinal >= ndev) return afail(nullptr, FP_E_INVALID, "device ordinal out of range");

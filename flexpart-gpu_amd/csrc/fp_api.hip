@@ -256,7 +256,7 @@ int fp_ctx_create(int device_ordinal, const fp_ctx_desc *desc, fp_ctx **out)
         return fail(nullptr, FP_E_INVALID, "met grid dimensions must be non-zero");
     if (d.met_nz > fp::kMaxLevels) return fail(nullptr, FP_E_INVALID, "met_nz exceeds 256");
     const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
-    if (npts * 2 >= 0x80000000ull) return fail(nullptr, FP_E_INVALID, "met grid too large");
+    if (npts > fp::kMaxMetPoints) return fail(nullptr, FP_E_INVALID, "met grid too large (at most 2^27 points)");
     if (d.n_species < 1 || d.n_species > FP_MAX_SPECIES)
         return fail(nullptr, FP_E_INVALID, "n_species must be 1..4");
     const uint64_t ncell = (uint64_t)d.out_nx * d.out_ny * d.out_nz;

@@ -257,7 +257,7 @@ int fp_ens_create(int device_ordinal, const fp_ens_desc *desc, fp_ens **out)
     }
     if (total == 0 || total > fp::kMaxSlots) return efail(nullptr, FP_E_INVALID, "total capacity must be in [1, 2^30]");
     const uint64_t npts = (uint64_t)d.met_nx * d.met_ny * d.met_nz;
-    if (npts * 2 >= 0x80000000ull) return efail(nullptr, FP_E_INVALID, "met grid too large");
+    if (npts > fp::kMaxMetPoints) return efail(nullptr, FP_E_INVALID, "met grid too large (at most 2^27 points)");
     if ((uint64_t)d.out_nx * d.out_ny * d.out_nz >= 0x80000000ull)
         return efail(nullptr, FP_E_INVALID, "output grid too large");
     int ndev = 0;

@@ -90,6 +90,29 @@ struct Soa {
 #endif
 };
 
+// Largest met grid (points) of a context, ensemble member or adjoint run:
+// the packed two-time bracket (32 B per point) stays below 4 GiB, so a gather
+// addresses it as its scalar base plus a 32-bit byte offset.
+constexpr uint64_t kMaxMetPoints = 1ull << 27;
+
+// The T at byte_off + imm bytes past a wavefront-uniform global base, loaded
+// in the scalar-base form (base in scalar registers, 32-bit offset in a
+// vector register, imm in the instruction: no 64-bit vector address
+// arithmetic).
+template <class T>
+__host__ __device__ __forceinline__ T load_at(const T *base, uint32_t byte_off, uint32_t imm = 0)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t p = reinterpret_cast<uint64_t>(base);
+    asm volatile("" : "+s"(p));
+    asm("" : "+v"(byte_off));
+    using G = const __attribute__((address_space(1))) char *;
+    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + byte_off + imm);
+#else
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off + imm);
+#endif
+}
+
 // Concentration-grid configuration shared by the fused and standalone paths.
 struct GridArgs {
     uint32_t *counts;

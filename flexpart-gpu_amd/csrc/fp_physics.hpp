@@ -155,8 +155,8 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
         float c[4][6];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float4 p = a.met[2u * idx[k]];
-            const float4 q = a.met[2u * idx[k] + 1u];
+            const float4 p = load_at(a.met, idx[k] << 5);
+            const float4 q = load_at(a.met, idx[k] << 5, 16u);
             c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z;
             c[k][3] = p.w; c[k][4] = q.x; c[k][5] = q.y;
         }

@@ -100,7 +100,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
             const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
             const uint32_t pidx = pi * a.ny + pj;
-            const float4 pbl0 = a.pbl[2u * pidx], pbl1 = a.pbl[2u * pidx + 1u];
+            const float4 pbl0 = load_at(a.pbl, pidx << 5), pbl1 = load_at(a.pbl, pidx << 5, 16u);
 
             // ---- Philox blocks per slot (langevin_fused.wgsl:430-451) and the
             // Box-Muller normals (they depend on the slot index only; drawn here,

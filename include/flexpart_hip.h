@@ -57,7 +57,7 @@ typedef struct fp_ctx fp_ctx;
  * the PBL grid is met_nx x met_ny (PblState, pbl/mod.rs:130-174). */
 typedef struct fp_ctx_desc {
     uint64_t capacity;      /* particle slots (ParticleStore::with_capacity), at most 2^30 */
-    uint32_t met_nx, met_ny, met_nz;
+    uint32_t met_nx, met_ny, met_nz; /* met_nx*met_ny*met_nz <= 2^27 (a 4 GiB packed bracket) */
     uint32_t n_species;     /* mass slots the step updates, 1..4 */
     uint32_t out_nx, out_ny, out_nz; /* concentration grid (0 = none) */
     uint32_t reserved[6];

@@ -63,3 +63,26 @@ def test_step_config_serialises_every_field(fp):
     assert c.dt_seconds == 900.0 and list(c.key) == [1, 2] and list(c.counter) == [3, 4, 5, 6]
     assert c.n_substeps == 2 and c.dry_mode == 1 and abs(c.dry_vd - 2e-3) < 1e-9 and c.grid_fused == 1
     assert np.isclose(c.min_height_m, 0.01)
+
+
+def test_met_grids_beyond_two_to_the_27_points_are_rejected(fp):
+    """Contexts, ensembles and adjoint runs address the packed met bracket (32 B
+    per point) with 32-bit byte offsets: a grid of 2^27 + a few points is
+    refused at creation, before any device call, with the limit in the
+    message."""
+    import ctypes as C
+    lib = fp.library()
+    big = dict(met_nx=2049, met_ny=1024, met_nz=64)   # 134,283,264 points > 2^27
+    h = C.c_void_p()
+    d = fp.CtxDesc(capacity=1, n_species=1, **big)
+    assert lib.fp_ctx_create(0, C.byref(d), C.byref(h)) == fp.FP_E_INVALID
+    assert b"2^27" in lib.fp_last_error(None)
+    caps = (C.c_uint64 * 2)(1, 1)
+    e = fp.EnsDesc(n_members=2, n_species=1, capacities=caps, **big)
+    assert lib.fp_ens_create(0, C.byref(e), C.byref(h)) == fp.FP_E_INVALID
+    assert b"2^27" in lib.fp_ens_last_error(None)
+    per = (C.c_uint32 * 1)(4)
+    a = fp.AdjDesc(n_sources=1, n_met_levels=2, obs_nz=2, obs_dz_m=100.0, n_steps=2, obs_first_step=1,
+                   checkpoint_every=1, particles_per_source=per, **big)
+    assert lib.fp_adj_create(0, C.byref(a), C.byref(h)) == fp.FP_E_INVALID
+    assert b"2^27" in lib.fp_adj_last_error(None)
