@@ -32,10 +32,8 @@ BYTES_PER_PARTICLE_STEP = 100  # 44 B state read + 56 B state and velocity writt
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_global_10m_traffic.md")
 VALU_FILE = os.path.join(ROOT, "profiles", "step_kernel_global_10m_valu.md")
-# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles
-SIMDS = 1024
-VALU_CYCLES = 4
-CLOCK_HZ = 2.4e9
+RATES_FILE = os.path.join(ROOT, "profiles", "valu_issue_rates_gfx950.md")
+SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
 
 
 def _package():
@@ -276,13 +274,23 @@ def main(argv=None) -> int:
     achieved_wall = n * BYTES_PER_PARTICLE_STEP / (elapsed / args.steps) / 1e9
     traffic = _traffic_bytes()
     valu_row = _profile_figure(VALU_FILE, "valu_per_row")
+    fast_ns = _profile_figure(RATES_FILE, "fast_form_ns")
+    slow_ns = _profile_figure(RATES_FILE, "slow_form_ns")
     valu = None
-    if valu_row:
-        busy = valu_row * (n / 64.0) * VALU_CYCLES / SIMDS / CLOCK_HZ
-        valu = {"valu_per_64_particle_row": valu_row, "frac": busy / avg_s,
-                "note": "VALU issue time of the kernel's instructions (SQ_INSTS_VALU per row from the committed "
-                        "rocprofv3 summary) at 4 cycles each on 1024 SIMDs at 2.4 GHz, over the event-timed "
-                        "launch"}
+    if valu_row and fast_ns and slow_ns:
+        per_simd = valu_row * (n / 64.0) / SIMDS  # wavefront VALU instructions per SIMD per launch
+        valu = {"valu_per_64_particle_row": valu_row,
+                "ns_per_valu_per_simd": avg_s * 1e9 / per_simd,
+                "fast_form_ns": fast_ns, "slow_form_ns": slow_ns,
+                "frac_at_fast_rate": per_simd * fast_ns * 1e-9 / avg_s,
+                "frac_at_slow_rate": per_simd * slow_ns * 1e-9 / avg_s,
+                "note": "SQ_INSTS_VALU per 64-particle row (committed rocprofv3 summary of this workload) "
+                        "issued on 1024 SIMDs: the event-timed launch spends ns_per_valu_per_simd per "
+                        "instruction; the measured issue cost of single instruction forms on this GPU is "
+                        "fast_form_ns (f32 add/mul/fmaak on vector registers) to slow_form_ns (scalar-register "
+                        "operands, compares, selects, min/max, shifts, conversions, f64 and 64-bit integer "
+                        "forms), so all of the kernel's instructions at the fast rate would fill "
+                        "frac_at_fast_rate of the launch and all at the slow rate frac_at_slow_rate"}
     result = {
         "metric": METRIC,
         "value": n * args.steps / elapsed,

@@ -25,6 +25,10 @@ LIB = os.path.join(LIBDIR, "libflexpart_hip.so")
 
 SOURCES = ["fp_step.hip", "fp_aux.hip", "fp_sort.hip", "fp_api.hip", "fp_tune.hip", "fp_ens.hip", "fp_adj.hip", "fp_release.hip", "fp_surface.hip", "fp_output.hip", "fp_met.hip", "fp_static.hip"]
 
+# stand-alone instruction-issue probe (an executable, not part of the library)
+PROBE_SRC = "valu_rate_probe.hip"
+PROBE = os.path.join(LIBDIR, "valu_rate_probe")
+
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -63,8 +67,12 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC, *FLAGS, "-o", PROBE, os.path.join(CSRC, PROBE_SRC)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PROBE}")
     return LIB
 
 
