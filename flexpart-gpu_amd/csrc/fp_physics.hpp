@@ -294,7 +294,8 @@ __device__ __forceinline__ Dz pow_log_floor(Dz zeta, float y, bool use, double L
 // on L and the mixing height of its PBL cell only (langevin_fused.wgsl:236).
 struct TurbRegime {
     float ust, wst, ol, h;
-    bool neu, uns;  // neutral, unstable (else stable)
+    float sigu_cell;  // the cell's unstable sigma_u (second PBL record)
+    bool neu, uns;    // neutral, unstable (else stable)
 };
 template <class T> struct TurbT : TurbRegime {
     T sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
@@ -319,17 +320,21 @@ __device__ __forceinline__ bool neutral_regime(float ol, float h)
 // (fpm_powf's own evaluation, so the values are those of fpm_powf).
 template <class T> struct VRawT { T sigw, dsigwdz, tlw, d0, d2; };
 
-template <class T>
-__device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zeta, double log_1em3)
+// R: the regime of every active lane of the wavefront (0 neutral, 1 unstable,
+// 2 stable), or -1 for a wavefront that mixes regimes (selected per lane).
+// The arithmetic of each regime is the same in every instantiation.
+template <int R, class T>
+__device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T zeta, double log_1em3)
 {
     const float ust = t.ust, wst = t.wst, ol = t.ol, h = t.h;
-    const bool neu = t.neu, uns = t.uns;
+    const bool neu = R < 0 ? t.neu : R == 0;
+    const bool uns = R < 0 ? t.uns : R == 1;
     VRawT<T> v;
     // d0: neutral z/u* (corr), unstable z/L, stable -1.3 u* / h
     v.d0 = (uns || neu ? z : kon<T>(-1.3f * ust)) / (neu ? ust : (uns ? ol : h));
     // E: neutral e^{-2e-4 corr}, unstable e^{-5 zeta}; d2 below serves the
     // same two regimes, so a wavefront of stable particles skips both
-    const bool any_nu = __ballot(neu || uns) != 0ull;
+    const bool any_nu = R < 0 ? __ballot(neu || uns) != 0ull : R != 2;
     T E = kon<T>(0.0f);
     if (any_nu) E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
     T pa = kon<T>(0.0f), pb = kon<T>(0.0f);
@@ -371,6 +376,19 @@ __device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zet
     return v;
 }
 
+// The regime-selected evaluation: a wavefront whose active lanes share one
+// regime runs that regime's instantiation (no per-lane selects between the
+// regimes' operands), a mixed wavefront the per-lane one.
+template <class T>
+__device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zeta, double log_1em3)
+{
+    const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
+    if (n == live) return vertical_raw_r<0>(t, z, zeta, log_1em3);
+    if (u == live) return vertical_raw_r<1>(t, z, zeta, log_1em3);
+    if ((n | u) == 0ull) return vertical_raw_r<2>(t, z, zeta, log_1em3);
+    return vertical_raw_r<-1>(t, z, zeta, log_1em3);
+}
+
 // hanna_short_w: the floors of langevin_fused.wgsl:335-339
 template <class T>
 __device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, double log_1em3, T &sigw,
@@ -400,28 +418,23 @@ __device__ __forceinline__ float4 pbl_cell_terms_dev(float4 pb)
     return make_float4(ol, sigu, __uint_as_float(neu ? 0u : (uns ? 1u : 2u)), 0.0f);
 }
 
-// compute_hanna (langevin_fused.wgsl:224-293) from the cell's two records
-template <class T>
-__device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, double log_1em3)
+// compute_hanna (langevin_fused.wgsl:224-293) from the cell's two records,
+// for a wavefront in one regime R or a mixed one (R = -1), as vertical_raw_r
+template <int R, class T>
+__device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_1em3)
 {
-    TurbT<T> t;
-    t.ust = fmaxf(pb.x, USTAR_MIN);
-    t.wst = fmaxf(pb.y, 0.0f);
-    t.h = fmaxf(pb.z, H_MIN);
-    t.ol = pc.x;
-    const uint32_t regime = __float_as_uint(pc.z);
-    t.neu = regime == 0u;
-    t.uns = regime == 1u;
     const float ust = t.ust, h = t.h;
+    const bool neu = R < 0 ? t.neu : R == 0;
+    const bool uns = R < 0 ? t.uns : R == 1;
     const T z = tclamp(z_in, 0.0f, h);
     const T zeta = tclamp(z / h, 0.0f, 1.0f);
-    const VRawT<T> v = vertical_raw(t, z, zeta, log_1em3);
+    const VRawT<T> v = vertical_raw_r<R>(t, z, zeta, log_1em3);
     T sigu, sigv, tlu, tlv;
-    if (t.neu) {
+    if (neu) {
         sigu = SIGMA_FLOOR + 2.0f * ust * texp(-3.0e-4f * v.d0);
         sigv = v.sigw;
-    } else if (t.uns) {
-        sigu = kon<T>(pc.y);
+    } else if (uns) {
+        sigu = kon<T>(t.sigu_cell);
         sigv = sigu;
     } else {
         sigu = SIGMA_FLOOR + 2.0f * ust * (1.0f - zeta);
@@ -429,11 +442,11 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
     }
     // unstable and stable: 0.15 h / sigma_u; neutral: T_Lu = T_Lw
     T d5 = kon<T>(0.0f);
-    if (__ballot(!t.neu) != 0ull) d5 = (0.15f * h) / sigu;
-    if (t.neu) {
+    if (R < 0 ? __ballot(!neu) != 0ull : R != 0) d5 = (0.15f * h) / sigu;
+    if (neu) {
         tlu = v.d2;
         tlv = tlu;
-    } else if (t.uns) {
+    } else if (uns) {
         tlu = d5;
         tlv = tlu;
     } else {
@@ -455,6 +468,25 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
     t.sigu = sigu; t.sigv = sigv; t.sigw = sigw; t.dsigwdz = dsigwdz;
     t.tlu = tlu; t.tlv = tlv; t.tlw = tlw;
     return t;
+}
+
+template <class T>
+__device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, double log_1em3)
+{
+    TurbT<T> t;
+    t.ust = fmaxf(pb.x, USTAR_MIN);
+    t.wst = fmaxf(pb.y, 0.0f);
+    t.h = fmaxf(pb.z, H_MIN);
+    t.ol = pc.x;
+    t.sigu_cell = pc.y;
+    const uint32_t regime = __float_as_uint(pc.z);
+    t.neu = regime == 0u;
+    t.uns = regime == 1u;
+    const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
+    if (n == live) return hanna_full_r<0>(t, z_in, log_1em3);
+    if (u == live) return hanna_full_r<1>(t, z_in, log_1em3);
+    if ((n | u) == 0ull) return hanna_full_r<2>(t, z_in, log_1em3);
+    return hanna_full_r<-1>(t, z_in, log_1em3);
 }
 
 // ---------------------------------------------------------------- Langevin
