@@ -219,6 +219,8 @@ void fp::fill_step_scalars(fp::StepArgs &a, const fp_step_params &p)
     a.wet_fraction = p.wet_fraction;
     a.wet_a = p.wet_a;
     a.wet_b = p.wet_b;
+    a.dry_sv = fp::dry_survival(p.dry_vd, p.dt_seconds, p.dry_href);
+    a.wet_sv = fp::wet_survival(p.wet_lambda, p.wet_fraction, p.dt_seconds);
     a.loss_mode = p.loss_mode;
     a.loss_min_mass = p.loss_min_mass;
     a.grid_fused = p.grid_fused ? 1u : 0u;

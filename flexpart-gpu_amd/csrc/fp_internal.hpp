@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "flexpart_hip.h"
+#include "fp_math.h"
 
 namespace fp {
 
@@ -95,6 +96,28 @@ struct Soa {
 // addresses it as its scalar base plus a 32-bit byte offset.
 constexpr uint64_t kMaxMetPoints = 1ull << 27;
 
+// Mass survival factors of one step, shared by the kernels (per-slot forcing)
+// and the host (uniform forcing, evaluated once per step): the same float
+// operations either way.
+// dry_deposition.wgsl:70-106, for a particle below 2 href
+__host__ __device__ inline float dry_survival(float vd, float dt, float href_in)
+{
+    const float href = fmaxf(href_in, 0.1f);
+    const float ex = -fmaxf(vd, 0.0f) * fabsf(dt) / (2.0f * href);
+    return fminf(fmaxf(fpm_expf(ex), 0.0f), 1.0f);
+}
+// wet_deposition.wgsl:73-111
+__host__ __device__ inline float wet_survival(float lam, float frac, float dt_in)
+{
+    const float l = fmaxf(lam, 0.0f);
+    const float dt = fabsf(dt_in);
+    const float fr = fminf(fmaxf(frac, 0.0f), 1.0f);
+    float prob = 0.0f;
+    if (!(l <= 0.0f || dt <= 0.0f || fr <= 0.0f))
+        prob = fminf(fmaxf((1.0f - fpm_expf(-l * dt)) * fr, 0.0f), 1.0f);
+    return 1.0f - prob;
+}
+
 // The T at byte_off + imm bytes past a wavefront-uniform global base, loaded
 // in the scalar-base form (base in scalar registers, 32-bit offset in a
 // vector register, imm in the instruction: no 64-bit vector address
@@ -154,6 +177,9 @@ struct StepArgs {
     float dry_vd, dry_href;
     uint32_t wet_mode;
     float wet_lambda, wet_fraction, wet_a, wet_b;
+    // survival factors of the uniform dry and wet modes (dry_survival and
+    // wet_survival of the scalars above, evaluated on the host)
+    float dry_sv, wet_sv;
     uint32_t loss_mode;
     float loss_min_mass;
     uint32_t grid_fused;

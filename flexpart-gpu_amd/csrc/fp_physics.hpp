@@ -505,7 +505,9 @@ __device__ __forceinline__ T ou_horizontal(T prev, T sigma_in, T tl_in, float dt
     const T tl = tpos(tl_in);
     const T r = dt / tl;
     const bool small = val(r) < 0.5f;
-    const T e = texp(-r);
+    // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
+    T e = kon<T>(0.0f);
+    if (__ballot(!small) != 0ull) e = texp(-r);
     const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
     return (val(tl) == 0.0f || !finite_f(val(nx))) ? kon<T>(0.0f) : nx;
@@ -519,7 +521,8 @@ __device__ __forceinline__ T ou_vertical(T prev, T sigw_in, T dsigwdz, T tlw_in,
     const T drift = sw * (dsigwdz + rho * sw);
     const T r = dt / tl;
     const bool small = val(r) < 0.5f;
-    const T e = texp(-r);
+    T e = kon<T>(0.0f);
+    if (__ballot(!small) != 0ull) e = texp(-r);
     const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
                  (small ? dt * drift : tl * (1.0f - e) * drift);

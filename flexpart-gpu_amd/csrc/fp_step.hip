@@ -142,9 +142,8 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.dry_mode != FP_DRY_OFF) {
                 const float href = fmaxf(a.dry_href, 0.1f);
                 if (!(pz >= 2.0f * href)) {
-                    const float vd = a.dry_mode == FP_DRY_PER_SLOT ? a.dry_vd_slots[at] : a.dry_vd;
-                    const float ex = -fmaxf(vd, 0.0f) * fabsf(a.dt) / (2.0f * href);
-                    const float sv = clampf(fpm_expf(ex), 0.0f, 1.0f);
+                    float sv = a.dry_sv;
+                    if (a.dry_mode == FP_DRY_PER_SLOT) sv = dry_survival(a.dry_vd_slots[at], a.dt, a.dry_href);
 #pragma unroll
                     for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
                         if (q < a.n_species) mass[q] = mass[q] * sv;
@@ -152,27 +151,16 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             }
             // ---- wet deposition (wet_deposition.wgsl:73-111)
             if (a.wet_mode != FP_WET_OFF) {
-                float lam, frac;
+                float sv = a.wet_sv;
                 if (a.wet_mode == FP_WET_PER_SLOT) {
-                    lam = a.wet_lambda_slots[at];
-                    frac = a.wet_fraction_slots[at];
+                    sv = wet_survival(a.wet_lambda_slots[at], a.wet_fraction_slots[at], a.dt);
                 } else if (a.wet_mode == FP_WET_PRECIP_FIELD) {
                     // below_cloud_scavenging_coefficient_gas (physics/wet_scavenging.rs:213-225)
                     const float pr = pos_or_zero(a.precip[pidx]);
-                    lam = 0.0f;
+                    float lam = 0.0f;
                     if (pr > 0.0f) lam = pos_or_zero(pos_or_zero(a.wet_a) * lds_powf(pr, pos_or_zero(a.wet_b)));
-                    frac = a.wet_fraction;
-                } else {
-                    lam = a.wet_lambda;
-                    frac = a.wet_fraction;
+                    sv = wet_survival(lam, a.wet_fraction, a.dt);
                 }
-                const float l = fmaxf(lam, 0.0f);
-                const float dt = fabsf(a.dt);
-                const float fr = clampf(frac, 0.0f, 1.0f);
-                float prob = 0.0f;
-                if (!(l <= 0.0f || dt <= 0.0f || fr <= 0.0f))
-                    prob = clampf((1.0f - fpm_expf(-l * dt)) * fr, 0.0f, 1.0f);
-                const float sv = 1.0f - prob;
 #pragma unroll
                 for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
                     if (q < a.n_species) mass[q] = mass[q] * sv;
