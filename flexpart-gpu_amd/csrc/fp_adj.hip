@@ -96,7 +96,7 @@ __device__ __forceinline__ WindGrad sample_wind_grad(const StepArgs &a, const fl
     const Axis ay = axis_of(y, a.ny);
     float lev, dlev;
     level_grad(z, lh, a.nz, a.has_heights != 0, lev, dlev);
-    const Axis az = axis_of(lev, a.nz);
+    const Axis az = a.has_heights ? axis_of(lev, a.nz) : axis_of<true>(lev, a.nz);
     const uint32_t ny = a.ny, nz = a.nz;
     const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - alpha;
     float gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};

@@ -22,8 +22,16 @@ constexpr float H_MIN = 1.0f;
 constexpr float F32_MAX = 3.40282347e38f;
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
-__device__ __forceinline__ bool finite_f(float v) { return (v == v) && (fabsf(v) <= F32_MAX); }
-__device__ __forceinline__ float pos_or_zero(float v) { return (finite_f(v) && v > 0.0f) ? v : 0.0f; }
+// clampf of a value produced by arithmetic (never a signaling NaN), lo <= hi,
+// as one median-of-three: for a quiet NaN x, v_med3_f32 returns the minimum
+// of its operands that are numbers, lo, which is what fminf(fmaxf(x, lo), hi)
+// gives; otherwise both are the middle value.  One instruction instead of a
+// max and a min.
+__device__ __forceinline__ float clampq(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+// Floating-point class tests (v_cmp_class_f32): bit 7 positive subnormal, bit 8
+// positive normal; 0x1f8 = every finite class (normals, subnormals, zeros).
+__device__ __forceinline__ bool finite_f(float v) { return __builtin_amdgcn_classf(v, 0x1f8); }
+__device__ __forceinline__ float pos_or_zero(float v) { return __builtin_amdgcn_classf(v, 0x180) ? v : 0.0f; }
 // a*(1-t) + b*t with the product-sum fused, as the reference CPU twin does
 // (a.mul_add(1.0 - t, b * t), physics/interpolation.rs:76-79); u = 1 - t.
 __device__ __forceinline__ float mix(float a, float b, float t, float u) { return __builtin_fmaf(a, u, b * t); }
@@ -87,8 +95,10 @@ __device__ __forceinline__ float unit24(uint32_t v) { return (float)(v >> 8) * (
 // box_muller_normals (langevin_fused.wgsl:186-192)
 __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, float &g1)
 {
-    const float ru = clampf(unit24(a), 1.1754944e-38f, 0.99999994f);
-    const float au = clampf(unit24(b), 0.0f, 0.99999994f);
+    // unit24 lies in [0, 1 - 2^-24] and 0.99999994f is 1 - 2^-24, so the
+    // reference's upper clamps (and the lower one of au) leave it unchanged
+    const float ru = fmaxf(unit24(a), 1.1754944e-38f);
+    const float au = unit24(b);
     const float rad = sqrt_cr(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1)
     float sn, cs;
     fpm_sincosf(6.2831855f * au, &sn, &cs);
@@ -100,9 +110,12 @@ __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, fl
 
 struct Axis { uint32_t lo, hi; float t; };
 
+// coord from arithmetic (clampq); axis_of<true> for a coordinate that may be
+// a value loaded from memory as it is (clampf)
+template <bool RAW = false>
 __device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
 {
-    const float c = clampf(coord, 0.0f, (float)(n - 1));
+    const float c = RAW ? clampf(coord, 0.0f, (float)(n - 1)) : clampq(coord, 0.0f, (float)(n - 1));
     const float fl = floorf(c);
     const uint32_t l = (uint32_t)fl;
     return Axis{l, min(l + 1u, n - 1u), c - fl};
@@ -120,7 +133,7 @@ __device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz,
     if (z <= h0) return h0 > 0.0f ? z / h0 : 0.0f;
     const float inv_w = lh[nz];
     const uint16_t *first = reinterpret_cast<const uint16_t *>(lh + nz + 1);
-    const uint32_t b = (uint32_t)fminf(fmaxf(z * inv_w, 0.0f), (float)(kLevelBuckets - 1u));
+    const uint32_t b = (uint32_t)clampq(z * inv_w, 0.0f, (float)(kLevelBuckets - 1u));
     uint32_t lo = first[b];
     while (lo < nz && !(z <= lh[lo])) ++lo;
     if (lo >= nz) return (float)(nz - 1);
@@ -139,7 +152,10 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
 {
     const Axis ax = axis_of(x, a.nx);
     const Axis ay = axis_of(y, a.ny);
-    const Axis az = axis_of(level_of(z, lh, a.nz, a.has_heights != 0), a.nz);
+    // with heights, level_of returns the result of arithmetic; in level-index
+    // mode it returns z as it is
+    const float lz = level_of(z, lh, a.nz, a.has_heights != 0);
+    const Axis az = a.has_heights ? axis_of(lz, a.nz) : axis_of<true>(lz, a.nz);
     const uint32_t ny = a.ny, nz = a.nz;
     const uint32_t r00 = (ax.lo * ny + ay.lo) * nz, r10 = (ax.hi * ny + ay.lo) * nz;
     const uint32_t r01 = (ax.lo * ny + ay.hi) * nz, r11 = (ax.hi * ny + ay.hi) * nz;
@@ -172,7 +188,7 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
 
 __device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
 {
-    const float cl = clampf(c, 0.0f, (float)(n - 1));
+    const float cl = clampq(c, 0.0f, (float)(n - 1));
     uint32_t ci = (uint32_t)floorf(cl);
     float fr = cl - (float)ci;
     if (ci >= n - 1u) { ci = n - 1u; fr = 0.0f; }
@@ -239,10 +255,11 @@ __device__ __forceinline__ Dz tmax(Dz x, float lo)
     const float r = fmaxf(x.v, lo);
     return Dz{r, r == x.v ? x.d : 0.0f};
 }
-__device__ __forceinline__ float tclamp(float x, float lo, float hi) { return clampf(x, lo, hi); }
+// (the profiles clamp heights and height ratios produced by arithmetic)
+__device__ __forceinline__ float tclamp(float x, float lo, float hi) { return clampq(x, lo, hi); }
 __device__ __forceinline__ Dz tclamp(Dz x, float lo, float hi)
 {
-    return Dz{clampf(x.v, lo, hi), (x.v > lo && x.v < hi) ? x.d : 0.0f};
+    return Dz{clampq(x.v, lo, hi), (x.v > lo && x.v < hi) ? x.d : 0.0f};
 }
 __device__ __forceinline__ float tpos(float x) { return pos_or_zero(x); }
 __device__ __forceinline__ Dz tpos(Dz x) { return (finite_f(x.v) && x.v > 0.0f) ? x : Dz{0.0f, 0.0f}; }
@@ -497,12 +514,15 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
 //   r >= 0.5: e^{-r} u + eta sigma sqrt(1-e^{-2r})  [+ T_L (1-e^{-r}) drift]
 // are one expression with selected coefficients: one division, one exp and
 // one sqrt per update whichever branch a lane takes.
+//
+// Precondition: sigma and T_L come from compute_hanna / hanna_short_w after
+// their floors (sigma finite and > 0, T_L finite and >= its 10 s or 30 s
+// floor).  The reference's clamps of both to [0, inf) and its T_L == 0 case
+// are then the identity and false, and are not evaluated.
 
 template <class T>
-__device__ __forceinline__ T ou_horizontal(T prev, T sigma_in, T tl_in, float dt, float eta)
+__device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, float eta)
 {
-    const T sigma = tpos(sigma_in);
-    const T tl = tpos(tl_in);
     const T r = dt / tl;
     const bool small = val(r) < 0.5f;
     // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
@@ -510,14 +530,12 @@ __device__ __forceinline__ T ou_horizontal(T prev, T sigma_in, T tl_in, float dt
     if (__ballot(!small) != 0ull) e = texp(-r);
     const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
-    return (val(tl) == 0.0f || !finite_f(val(nx))) ? kon<T>(0.0f) : nx;
+    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
 
 template <class T>
-__device__ __forceinline__ T ou_vertical(T prev, T sigw_in, T dsigwdz, T tlw_in, float dt, float rho, float eta)
+__device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt, float rho, float eta)
 {
-    const T sw = tpos(sigw_in);
-    const T tl = tpos(tlw_in);
     const T drift = sw * (dsigwdz + rho * sw);
     const T r = dt / tl;
     const bool small = val(r) < 0.5f;
@@ -526,7 +544,7 @@ __device__ __forceinline__ T ou_vertical(T prev, T sigw_in, T dsigwdz, T tlw_in,
     const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
                  (small ? dt * drift : tl * (1.0f - e) * drift);
-    return (val(tl) == 0.0f || !finite_f(val(nx))) ? kon<T>(0.0f) : nx;
+    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
 
 // PBL reflection (langevin_fused.wgsl:481-500)
@@ -539,7 +557,7 @@ __device__ __forceinline__ void reflect(float &pz, float &tw, int32_t &cbt, floa
         else if (z > h) { z = 2.0f * h - z; cbt = -cbt; tw = -tw; }
         else break;
     }
-    pz = clampf(z, zmin, h);
+    pz = clampq(z, zmin, h);
 }
 
 // ---------------------------------------------------------------- gridding

@@ -83,22 +83,22 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const float x0 = (float)cx + px, y0 = (float)cy + py, z0 = pz;
             const Wind s0 = sample_wind(a, lh, alpha, x0, y0, z0);
             const float u0 = s0.u + tu, v0 = s0.v + tv, w0 = s0.w;
-            const float qx = clampf(x0 + (u0 * a.x_scale) * a.dt_adv, 0.0f, nxm);
-            const float qy = clampf(y0 + (v0 * a.y_scale) * a.dt_adv, 0.0f, nym);
-            const float qz = clampf(z0 + (w0 * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
+            const float qx = clampq(x0 + (u0 * a.x_scale) * a.dt_adv, 0.0f, nxm);
+            const float qy = clampq(y0 + (v0 * a.y_scale) * a.dt_adv, 0.0f, nym);
+            const float qz = clampq(z0 + (w0 * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
             const Wind s1 = sample_wind(a, lh, alpha, qx, qy, qz);
             const float u1 = s1.u + tu, v1 = s1.v + tv, w1 = s1.w;
             const float cu = 0.5f * (u0 + u1), cv = 0.5f * (v0 + v1), cw = 0.5f * (w0 + w1);
-            const float x1 = clampf(x0 + (cu * a.x_scale) * a.dt_adv, 0.0f, nxm);
-            const float y1 = clampf(y0 + (cv * a.y_scale) * a.dt_adv, 0.0f, nym);
-            pz = clampf(z0 + (cw * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
+            const float x1 = clampq(x0 + (cu * a.x_scale) * a.dt_adv, 0.0f, nxm);
+            const float y1 = clampq(y0 + (cv * a.y_scale) * a.dt_adv, 0.0f, nym);
+            pz = clampq(z0 + (cw * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
             split_cell(x1, a.nx, cx, px);
             split_cell(y1, a.ny, cy, py);
 
             // ---- PBL lookup + Hanna (langevin_fused.wgsl:414-427)
             const float hx = (float)cx + px, hy = (float)cy + py;
-            const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
-            const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
+            const uint32_t pi = (uint32_t)floorf(clampq(hx, 0.0f, nxm));
+            const uint32_t pj = (uint32_t)floorf(clampq(hy, 0.0f, nym));
             const uint32_t pidx = pi * a.ny + pj;
             const float4 pbl0 = load_at(a.pbl, pidx << 5), pbl1 = load_at(a.pbl, pidx << 5, 16u);
 

@@ -49,6 +49,20 @@ def test_unary_primitives_match_cpu_bitwise(ctx, orc, fn):
     assert ok, f"{fn}: {bad} of {x.size} differ"
 
 
+def test_log_reduction_division_exhaustive(ctx, orc):
+    """Every float in [0.5, 1) (and its negative-exponent copy in the unit
+    log): together they reach every reduced mantissa m in [sqrt(1/2), sqrt(2)),
+    hence every operand pair of the log's f / (2 + f), which the device
+    evaluates without the division's range scaling."""
+    x = np.arange(0x3F000000, 0x3F800000, dtype=np.uint32).view(np.float32)
+    for fn in ("log", "log_unit"):
+        ok, bad = bits_equal(ctx.math_probe(fn, x), orc.math_eval("log", x))
+        assert ok, f"{fn}: {bad} of {x.size} differ"
+    y = (x * np.float32(2.0 ** -100)).astype(np.float32)
+    ok, bad = bits_equal(ctx.math_probe("log_unit", y), orc.math_eval("log", y))
+    assert ok, f"log_unit at 2^-100 scale: {bad} differ"
+
+
 def test_pow_matches_cpu_bitwise(ctx, orc):
     rng = np.random.default_rng(12)
     xs, ys = [], []
