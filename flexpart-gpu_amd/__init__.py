@@ -228,6 +228,7 @@ ABI = {
     "fp_sort_morton": (C.c_int, [_vp, C.POINTER(SortParams), _u32]),
     "fp_philox_blocks": (C.c_int, [_vp, _u32, _u32, C.c_uint64, _u32]),
     "fp_math_probe": (C.c_int, [_vp, C.c_int, _f, _f, _f, C.c_uint64]),
+    "fp_math_sweep": (C.c_int, [_vp, C.c_int, C.c_uint32, C.c_uint32, _u64, _u32]),
     "fp_profile_enable": (C.c_int, [_vp, C.c_int]),
     "fp_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fp_profile_reset": (C.c_int, [_vp]),
@@ -660,6 +661,18 @@ class GpuTimestep:
         self._chk(self._lib.fp_math_probe(self._h, self.PROBES[fn], _fp(xs), _fp(ys), _fp(out),
                                           xs.size))
         return out
+
+    SWEEPS = {"sqrt_step": 0}
+
+    def math_sweep(self, fn: str, first_bits: int, last_bits: int) -> tuple:
+        """Exhaustive device check of a step primitive over float bit patterns
+        [first_bits, last_bits] (fp_math_sweep): (failures, lowest failing
+        bit pattern or 0xffffffff)."""
+        fails = np.zeros(1, np.uint64)
+        first = np.zeros(1, np.uint32)
+        self._chk(self._lib.fp_math_sweep(self._h, self.SWEEPS[fn], int(first_bits), int(last_bits),
+                                          fails.ctypes.data_as(_u64), first.ctypes.data_as(_u32)))
+        return int(fails[0]), int(first[0])
 
     # -- timing / tuning
     def profile(self, enable: bool = True):

@@ -1059,6 +1059,28 @@ int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *ou
     return FP_OK;
 }
 
+int fp_math_sweep(fp_ctx *ctx, int fn, uint32_t first_bits, uint32_t last_bits, uint64_t *failures,
+                  uint32_t *first_failure)
+{
+    FP_CHECK(ctx, ctx && failures && first_failure, "null argument");
+    FP_CHECK(ctx, fn == FP_SWEEP_SQRT_STEP, "unknown sweep function");
+    FP_CHECK(ctx, first_bits <= last_bits, "empty bit-pattern range");
+    uint32_t *d = nullptr;
+    FP_HIP(ctx, dalloc(&d, 4));
+    auto *fails = reinterpret_cast<unsigned long long *>(d);
+    hipError_t e = fp::launch_math_sweep(fn, first_bits, last_bits, fails, d + 2, ctx->stream);
+    unsigned long long hf = 0;
+    uint32_t hw = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hf, fails, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hw, d + 2, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    FP_HIP(ctx, e);
+    *failures = hf;
+    *first_failure = hw;
+    return FP_OK;
+}
+
 // ---------------------------------------------------------------- timing / tuning
 
 int fp_profile_enable(fp_ctx *ctx, int enable)

@@ -248,6 +248,48 @@ __global__ void math_probe_kernel(int fn, const float *x, const float *y, float 
     out[i] = r;
 }
 
+// Exhaustive check of the step's square root (sqrt_in_range) over the float
+// bit patterns [first, last]: the result s is the correctly rounded root of x
+// exactly when x lies strictly between the squares of the midpoints from s to
+// its two float neighbours.  Those midpoints have at most 26 significant bits,
+// so their binary64 squares are exact and the test involves no rounding (a
+// float x never equals such a square, which has more than 24 significant bits).
+constexpr uint32_t kSweepPerThread = 64;
+__global__ void sqrt_sweep_kernel(uint32_t first, uint32_t last, unsigned long long *fails, uint32_t *first_fail)
+{
+    const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kSweepPerThread + first;
+    unsigned long long bad = 0;
+    uint32_t where = 0xffffffffu;
+    for (uint32_t k = 0; k < kSweepPerThread; ++k) {
+        const uint64_t u = i0 + k;
+        if (u > last) break;
+        const float x = __uint_as_float((uint32_t)u);
+        const float s = sqrt_in_range(x);
+        const uint32_t sb = __float_as_uint(s);
+        const double sd = (double)s;
+        const double lo = 0.5 * (sd + (double)__uint_as_float(sb - 1u));
+        const double hi = 0.5 * (sd + (double)__uint_as_float(sb + 1u));
+        const double xd = (double)x;
+        const bool ok = sb > 0x00800000u && sb < 0x7f800000u && lo * lo < xd && xd < hi * hi;
+        if (!ok) {
+            ++bad;
+            where = min(where, (uint32_t)u);
+        }
+    }
+    if (bad) {
+        atomicAdd(fails, bad);
+        atomicMin(first_fail, where);
+    }
+}
+
+__global__ void sweep_init_kernel(unsigned long long *fails, uint32_t *first_fail)
+{
+    if (threadIdx.x == 0) {
+        *fails = 0ull;
+        *first_fail = 0xffffffffu;
+    }
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -336,6 +378,16 @@ hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out,
 {
     if (n == 0) return hipSuccess;
     math_probe_kernel<<<blocks_for(n, 256), 256, 0, st>>>(fn, x, y, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_math_sweep(int fn, uint32_t first, uint32_t last, unsigned long long *fails,
+                             uint32_t *first_fail, hipStream_t st)
+{
+    if (fn != FP_SWEEP_SQRT_STEP || first > last) return hipErrorInvalidValue;
+    sweep_init_kernel<<<1, 64, 0, st>>>(fails, first_fail);
+    const uint64_t n = (uint64_t)last - first + 1u, per_block = 256ull * kSweepPerThread;
+    sqrt_sweep_kernel<<<(uint32_t)((n + per_block - 1) / per_block), 256, 0, st>>>(first, last, fails, first_fail);
     return hipGetLastError();
 }
 

@@ -381,6 +381,8 @@ hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2]
                                 uint32_t *out, hipStream_t st);
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
                              hipStream_t st);
+hipError_t launch_math_sweep(int fn, uint32_t first, uint32_t last, unsigned long long *fails,
+                             uint32_t *first_fail, hipStream_t st);
 // Source-term injection (fp_release.hip).
 hipError_t release_plan_and_launch(const fp_grid_domain &g, const fp_release *rel, uint32_t n_rel,
                                    uint64_t *released, int64_t now, int32_t time, const uint32_t *slots_dev,

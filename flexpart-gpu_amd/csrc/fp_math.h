@@ -71,11 +71,25 @@ FPM_FN float fpm_div_unscaled(float n, float d)
 #endif
 }
 
+/* min(max(x, lo), hi) for lo <= hi.  On the GPU one median-of-three
+ * (v_med3_f32), which gives the same value: for a quiet NaN x it returns lo,
+ * as min(max(x, lo), hi) does.  (Callers here discard the clamped value of a
+ * NaN argument, so a signaling NaN, for which the two differ, changes no
+ * result either.) */
+FPM_FN float fpm_clampf(float x, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+    return __builtin_fminf(__builtin_fmaxf(x, lo), hi);
+#endif
+}
+
 /* e^x, single precision.  Special cases are selected at the end rather than
  * branched on, so a wavefront never splits inside the function. */
 FPM_FN float fpm_expf(float x)
 {
-    const float xc = __builtin_fminf(__builtin_fmaxf(x, -104.0f), 89.0f);
+    const float xc = fpm_clampf(x, -104.0f, 89.0f);
     const float k = __builtin_rintf(xc * 1.44269502162933349609375f);
     /* ln2 = 0.693145751953125 (exact, 12 trailing zero bits) + 1.4286068e-6 */
     float r = __builtin_fmaf(k, -0.693145751953125f, xc);
@@ -91,7 +105,9 @@ FPM_FN float fpm_expf(float x)
     /* ldexp rounds y * 2^k once, exactly like fpm_scale2; it overflows to
      * +inf for x in (88.72, 89]. */
     const float res = __builtin_ldexpf(1.0f + p, (int)k);
-    return x != x ? x : (x < -104.0f ? 0.0f : res);
+    /* every x below -104 (and -inf) is evaluated at -104, whose result is 0
+     * (e^-104 is below half the smallest subnormal); only NaN is selected */
+    return x != x ? x : res;
 }
 
 /* Natural logarithm, single precision (special cases selected, not branched). */

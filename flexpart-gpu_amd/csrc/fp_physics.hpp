@@ -36,23 +36,37 @@ __device__ __forceinline__ float pos_or_zero(float v) { return __builtin_amdgcn_
 // (a.mul_add(1.0 - t, b * t), physics/interpolation.rs:76-79); u = 1 - t.
 __device__ __forceinline__ float mix(float a, float b, float t, float u) { return __builtin_fmaf(a, u, b * t); }
 
-// Correctly rounded square root.  For x >= 2^-96 this is the compiler's IEEE
-// sequence (hardware square root, then the choice between s - 1 ulp, s and
-// s + 1 ulp by two fused residuals) without the rescaling of arguments below
-// 2^-96 and the class check for zero and infinity, which change nothing in
-// that range; a wavefront with any lane below 2^-96 (or zero, negative or
-// NaN) takes the full sequence.  The step's square roots (Box-Muller radius,
-// Ornstein-Uhlenbeck noise scale, unstable sigma_w) have positive normal
-// arguments, so the short path is the one taken.
+// Correctly rounded square root of x in [2^-96, FLT_MAX]: the hardware
+// square root s0 and reciprocal square root y of x, then one correction
+// s0 + (x - s0^2) * (y / 2) with the residual x - s0^2 exact in a fused
+// multiply-add.  Every float in that range gives the IEEE result; this was
+// checked exhaustively on gfx950 by an exact binary64 test of the rounding
+// (fp_math_sweep).  Below 2^-96 the residual underflows and the result can be
+// off by one unit in the last place.
+__device__ __forceinline__ float sqrt_in_range(float x)
+{
+    const float s0 = __builtin_amdgcn_sqrtf(x);
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float d = __builtin_fmaf(-s0, s0, x);
+    return __builtin_fmaf(d, 0.5f * y, s0);
+}
+
+// Correctly rounded square root of any x: a wavefront with a lane outside
+// [2^-96, FLT_MAX] (zero, tiny, negative, infinite or NaN) takes the
+// compiler's IEEE sequence.
 __device__ __forceinline__ float sqrt_cr(float x)
 {
+    if (__builtin_expect(__ballot(!(x >= 0x1p-96f && x <= F32_MAX)) != 0ull, 0)) return __builtin_sqrtf(x);
+    return sqrt_in_range(x);
+}
+
+// sqrt_cr of an x known not to be +infinity: only the lower bound is tested.
+// The step's Box-Muller radii (-2 log u, u in [2^-126, 1)) and
+// Ornstein-Uhlenbeck noise scales (arguments in [0, 1]) are such arguments.
+__device__ __forceinline__ float sqrt_cr_finite(float x)
+{
     if (__builtin_expect(__ballot(!(x >= 0x1p-96f)) != 0ull, 0)) return __builtin_sqrtf(x);
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
-    r = __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
-    return r;
+    return sqrt_in_range(x);
 }
 
 // ---------------------------------------------------------------- Philox
@@ -99,7 +113,7 @@ __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, fl
     // reference's upper clamps (and the lower one of au) leave it unchanged
     const float ru = fmaxf(unit24(a), 1.1754944e-38f);
     const float au = unit24(b);
-    const float rad = sqrt_cr(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1)
+    const float rad = sqrt_cr_finite(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1): at most 175
     float sn, cs;
     fpm_sincosf(6.2831855f * au, &sn, &cs);
     g0 = rad * cs;
@@ -245,6 +259,13 @@ __device__ __forceinline__ float tsqrt(float x) { return sqrt_cr(x); }
 __device__ __forceinline__ Dz tsqrt(Dz x)
 {
     const float r = sqrt_cr(x.v);
+    return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
+}
+// (of an argument known not to be +infinity)
+__device__ __forceinline__ float tsqrt_finite(float x) { return sqrt_cr_finite(x); }
+__device__ __forceinline__ Dz tsqrt_finite(Dz x)
+{
+    const float r = sqrt_cr_finite(x.v);
     return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
 }
 __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
@@ -467,7 +488,7 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
         tlu = d5;
         tlv = tlu;
     } else {
-        tlu = d5 * tsqrt(zeta);
+        tlu = d5 * tsqrt_finite(zeta);  // zeta in [0, 1]
         tlv = 0.467f * tlu;
     }
     T sigw = v.sigw, dsigwdz = v.dsigwdz, tlw = v.tlw;
@@ -528,7 +549,7 @@ __device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, floa
     // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
     T e = kon<T>(0.0f);
     if (__ballot(!small) != 0ull) e = texp(-r);
-    const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
+    const T root = tsqrt_finite(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
     return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
@@ -541,7 +562,7 @@ __device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt
     const bool small = val(r) < 0.5f;
     T e = kon<T>(0.0f);
     if (__ballot(!small) != 0ull) e = texp(-r);
-    const T root = tsqrt(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
+    const T root = tsqrt_finite(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
     const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
                  (small ? dt * drift : tl * (1.0f - e) * drift);
     return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;

@@ -444,6 +444,17 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
 #define FP_PROBE_LOG_UNIT 7
 int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n);
 
+/* Exhaustive device check of one of the step's arithmetic primitives over the
+ * float bit patterns [first_bits, last_bits]; *failures receives the number of
+ * inputs whose result is not the correctly rounded one and *first_failure the
+ * lowest such bit pattern (0xffffffff if none).  FP_SWEEP_SQRT_STEP: the
+ * step's square root of arguments in [2^-96, FLT_MAX], each result tested
+ * exactly in binary64.  Diagnostic; no counterpart in the reference.
+ * Blocking. */
+#define FP_SWEEP_SQRT_STEP 0
+int fp_math_sweep(fp_ctx *ctx, int fn, uint32_t first_bits, uint32_t last_bits, uint64_t *failures,
+                  uint32_t *first_failure);
+
 /* ---- timing ----------------------------------------------------------- */
 
 /* HIP-event timing of each kernel launch on the context stream (the

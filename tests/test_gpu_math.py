@@ -92,3 +92,19 @@ def test_sqrt_and_division_are_correctly_rounded(ctx):
         want = (a / b).astype(np.float32)
     ok, bad = bits_equal(ctx.math_probe("div", a, b), want)
     assert ok, f"div: {bad} differ"
+
+
+def test_step_sqrt_exhaustive(ctx):
+    """The step's square root (hardware sqrt and rsqrt, one fused correction)
+    is correctly rounded for every float in [2^-96, FLT_MAX]: each result is
+    checked on the device against the exact binary64 squares of the midpoints
+    to its neighbours."""
+    fails, first = ctx.math_sweep("sqrt_step", 0x0F800000, 0x7F7FFFFF)
+    assert fails == 0, f"{fails} incorrectly rounded, first at bits {first:#010x}"
+
+
+def test_step_sqrt_sweep_detects_misrounding(ctx):
+    """Below 2^-96 the residual of the same sequence underflows: the sweep
+    must report failures there (the exact test has teeth)."""
+    fails, first = ctx.math_sweep("sqrt_step", 0x00800000, 0x01000000)
+    assert fails > 0 and 0x00800000 <= first <= 0x01000000

@@ -73,6 +73,10 @@ def test_exp_log_special_values(orc):
     e = orc.math_eval("exp", x)
     assert e[0] == 1.0 and e[1] == 0.0 and e[2] == np.inf and e[4] == np.inf and e[5] == 0.0
     assert e[3] == np.float32(np.e)
+    # every argument at or below -104 (the clamp) gives exactly 0, NaN stays NaN
+    low = orc.math_eval("exp", np.array([-104.0, -104.5, -150.0, -1e30], np.float32))
+    assert (low.view(np.uint32) == 0).all()
+    assert np.isnan(orc.math_eval("exp", np.array([np.nan], np.float32)))[0]
     lg = orc.math_eval("log", np.array([1.0, 0.0, np.inf, 2.0], np.float32))
     assert lg[0] == 0.0 and lg[1] == -np.inf and lg[2] == np.inf
     assert lg[3] == np.float32(np.log(2.0))
