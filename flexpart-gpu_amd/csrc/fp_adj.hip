@@ -92,11 +92,11 @@ __device__ __forceinline__ WindGrad sample_wind_grad(const StepArgs &a, const fl
     WindGrad g;
     // sum over the 8 corners of d(weight)/d(coordinate) * value, corner by
     // corner (the step's own sample_wind read the same lines)
-    const Axis ax = axis_of(x, a.nx);
-    const Axis ay = axis_of(y, a.ny);
+    const Axis ax = axis_of(x, a.nx, a.nxm);
+    const Axis ay = axis_of(y, a.ny, a.nym);
     float lev, dlev;
     level_grad(z, lh, a.nz, a.has_heights != 0, lev, dlev);
-    const Axis az = a.has_heights ? axis_of(lev, a.nz) : axis_of<true>(lev, a.nz);
+    const Axis az = a.has_heights ? axis_of(lev, a.nz, a.nzm) : axis_of<true>(lev, a.nz, a.nzm);
     const uint32_t ny = a.ny, nz = a.nz;
     const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - alpha;
     float gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         const float alpha = a.alpha;
 
         // ---- forward recomputation (the step kernel's operations)
-        const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
+        const float nxm = a.nxm, nym = a.nym;
         const float x0 = (float)cx + px, y0 = (float)cy + py, z0 = pz;
         const Wind sw0 = sample_wind(a, lh, alpha, x0, y0, z0);
         const float u0 = sw0.u + tu, v0 = sw0.v + tv, w0 = sw0.w;
@@ -221,13 +221,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         const float z1 = clampf(fz, 0.0f, a.zmax);
         int32_t ncx, ncy;
         float npx, npy;
-        split_cell(x1, a.nx, ncx, npx);
-        split_cell(y1, a.ny, ncy, npy);
+        split_cell(x1, a.nx, nxm, ncx, npx);
+        split_cell(y1, a.ny, nym, ncy, npy);
         const float hx = (float)ncx + npx, hy = (float)ncy + npy;
         const uint32_t pi = (uint32_t)floorf(clampf(hx, 0.0f, nxm));
         const uint32_t pj = (uint32_t)floorf(clampf(hy, 0.0f, nym));
         const uint32_t pidx = pi * a.ny + pj;
-        const TurbT<Dz> t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], Dz{z1, 1.0f}, a.log_1em3);
+        const TurbT<Dz> t = hanna_full(a.pbl[2u * pidx], a.pbl[2u * pidx + 1u], Dz{z1, 1.0f}, a.log_1em3,
+                                           a.lean_div != 0u);
 
         const U4 ctr{a.c0, a.c1, a.c2, a.c3};
         const uint64_t cbase = i * a.blocks_per_particle;
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
             A[0] = ou_coef(t.tlw.v, a.dt);
             G[0] = twn.d;
         } else {
-            const float dts = a.dt / (float)nsub;
+            const float dts = a.dt_sub;  // a.dt / n_substeps
             const float zmin = a.min_height, h = t.h;
             Dz sigw = t.sigw, dsw = t.dsigwdz, tlw = t.tlw;
             float z = z1, w = tw;
@@ -325,7 +326,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
             lz = lz + G[0] * ltw;
             ltw = A[0] * ltw;
         } else {
-            const float dts = a.dt / (float)nsub;
+            const float dts = a.dt_sub;  // a.dt / n_substeps
 #pragma unroll
             for (int sub = (NSUB > 0 ? NSUB : 4) - 1; sub >= 0; --sub) {
                 if (NSUB < 0 && (uint32_t)sub >= nsub) continue;
@@ -642,7 +643,7 @@ __device__ __forceinline__ float survival_factor(const StepArgs &a, const Soa &s
         }
     }
     if (a.wet_mode != FP_WET_OFF) {
-        const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
+        const float nxm = a.nxm, nym = a.nym;
         const float hx = (float)s.i(F_CELL_X)[j] + s.f(F_POS_X)[j], hy = (float)s.i(F_CELL_Y)[j] + s.f(F_POS_Y)[j];
         const uint32_t pidx = (uint32_t)floorf(clampf(hx, 0.0f, nxm)) * a.ny + (uint32_t)floorf(clampf(hy, 0.0f, nym));
         float lam, frac;

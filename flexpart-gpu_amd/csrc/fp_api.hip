@@ -225,6 +225,15 @@ void fp::fill_step_scalars(fp::StepArgs &a, const fp_step_params &p)
     a.loss_min_mass = p.loss_min_mass;
     a.grid_fused = p.grid_fused ? 1u : 0u;
     a.log_1em3 = fpm_log_d((double)1.0e-3f);
+    // derived from the grid dimensions, which every caller sets first
+    a.nxm = (float)(a.nx - 1u);
+    a.nym = (float)(a.ny - 1u);
+    a.nzm = (float)(a.nz - 1u);
+    a.dt_sub = a.dt / (float)nv;
+    a.dry_zcut = 2.0f * std::fmax(a.dry_href, 0.1f);  // fmaxf semantics: 0.1 for a NaN href
+    auto in = [](float v, float lo, float hi) { return v >= lo && v <= hi; };
+    a.lean_div = in(a.dt, 0x1p-20f, 0x1p20f) && in(a.dt_sub, 0x1p-20f, 0x1p20f) && a.min_height >= 0x1p-40f &&
+                 a.min_height < 1.0f;
 }
 
 void fp::counter_advance(const uint32_t c[4], uint64_t off, uint32_t out[4])

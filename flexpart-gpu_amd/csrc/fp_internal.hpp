@@ -186,6 +186,14 @@ struct StepArgs {
     GridArgs grid;
     uint32_t ppt;           // particles per thread
     double log_1em3;        // fpm_log_d(1e-3f): the log of the unstable profile's zeta floor
+    // launch-uniform values the step would otherwise recompute in every lane
+    // (fill_step_scalars, with the same f32 operations): (float)(n - 1) per
+    // grid axis, the sub-step length dt / n_substeps and the dry-deposition
+    // height cut 2 max(href, 0.1)
+    float nxm, nym, nzm, dt_sub, dry_zcut;
+    // the launch's part of the range check for unscaled quotients in the
+    // turbulence (lean_lane): dt and dt_sub in [2^-20, 2^20], z_min in [2^-40, 1)
+    uint32_t lean_div;
     // gathered steps (launch_step_gathered): slot i of s receives the words
     // the step writes of src slot gather_map[i] advanced by one step, i.e. the
     // step of the state that a compaction with survivor map gather_map would

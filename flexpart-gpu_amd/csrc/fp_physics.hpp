@@ -126,13 +126,19 @@ struct Axis { uint32_t lo, hi; float t; };
 
 // coord from arithmetic (clampq); axis_of<true> for a coordinate that may be
 // a value loaded from memory as it is (clampf)
+// nm = (float)(n - 1), precomputed for the launch
 template <bool RAW = false>
-__device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
+__device__ __forceinline__ Axis axis_of(float coord, uint32_t n, float nm)
 {
-    const float c = RAW ? clampf(coord, 0.0f, (float)(n - 1)) : clampq(coord, 0.0f, (float)(n - 1));
+    const float c = RAW ? clampf(coord, 0.0f, nm) : clampq(coord, 0.0f, nm);
     const float fl = floorf(c);
     const uint32_t l = (uint32_t)fl;
     return Axis{l, min(l + 1u, n - 1u), c - fl};
+}
+template <bool RAW = false>
+__device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
+{
+    return axis_of<RAW>(coord, n, (float)(n - 1));
 }
 
 // z [m] -> fractional level: the first k >= 1 with z <= h[k] (the linear scan
@@ -164,12 +170,12 @@ struct Wind { float u, v, w; };
 __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, float alpha, float x,
                                             float y, float z)
 {
-    const Axis ax = axis_of(x, a.nx);
-    const Axis ay = axis_of(y, a.ny);
+    const Axis ax = axis_of(x, a.nx, a.nxm);
+    const Axis ay = axis_of(y, a.ny, a.nym);
     // with heights, level_of returns the result of arithmetic; in level-index
     // mode it returns z as it is
     const float lz = level_of(z, lh, a.nz, a.has_heights != 0);
-    const Axis az = a.has_heights ? axis_of(lz, a.nz) : axis_of<true>(lz, a.nz);
+    const Axis az = a.has_heights ? axis_of(lz, a.nz, a.nzm) : axis_of<true>(lz, a.nz, a.nzm);
     const uint32_t ny = a.ny, nz = a.nz;
     const uint32_t r00 = (ax.lo * ny + ay.lo) * nz, r10 = (ax.hi * ny + ay.lo) * nz;
     const uint32_t r01 = (ax.lo * ny + ay.hi) * nz, r11 = (ax.hi * ny + ay.hi) * nz;
@@ -200,14 +206,18 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
     return Wind{mix(v[0], v[3], alpha, ua), mix(v[1], v[4], alpha, ua), mix(v[2], v[5], alpha, ua)};
 }
 
-__device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
+__device__ __forceinline__ void split_cell(float c, uint32_t n, float nm, int32_t &cell, float &frac)
 {
-    const float cl = clampq(c, 0.0f, (float)(n - 1));
+    const float cl = clampq(c, 0.0f, nm);
     uint32_t ci = (uint32_t)floorf(cl);
     float fr = cl - (float)ci;
     if (ci >= n - 1u) { ci = n - 1u; fr = 0.0f; }
     cell = (int32_t)ci;
     frac = fr;
+}
+__device__ __forceinline__ void split_cell(float c, uint32_t n, int32_t &cell, float &frac)
+{
+    split_cell(c, n, (float)(n - 1), cell, frac);
 }
 
 // ---------------------------------------------------------------- height derivatives
@@ -284,6 +294,18 @@ __device__ __forceinline__ Dz tclamp(Dz x, float lo, float hi)
 }
 __device__ __forceinline__ float tpos(float x) { return pos_or_zero(x); }
 __device__ __forceinline__ Dz tpos(Dz x) { return (finite_f(x.v) && x.v > 0.0f) ? x : Dz{0.0f, 0.0f}; }
+// Quotients of the Hanna profiles and the Langevin updates.  LEAN: the
+// operands are known to lie in the range where fpm_div_unscaled equals the
+// IEEE division (TurbRegime::lean); the height-derivative type keeps the
+// IEEE division, whose value part is then the same.
+template <bool LEAN> __device__ __forceinline__ float qdiv(float n, float d)
+{
+    return LEAN ? fpm_div_unscaled(n, d) : n / d;
+}
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, Dz d) { return n / d; }
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, float d) { return n / d; }
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(float n, Dz d) { return n / d; }
+
 // The binary64 log/exp tables of fp_math.h, copied into LDS by the kernels
 // that evaluate the Hanna powers (load_math_tab, before the kernel's first
 // barrier): the same values as the device constants, read at LDS latency
@@ -334,6 +356,10 @@ struct TurbRegime {
     float ust, wst, ol, h;
     float sigu_cell;  // the cell's unstable sigma_u (second PBL record)
     bool neu, uns;    // neutral, unstable (else stable)
+    // wavefront-uniform: every lane's cell and height are in the range where
+    // each quotient of the profiles and of the Langevin updates needs no
+    // range scaling (lean_lane)
+    bool lean;
 };
 template <class T> struct TurbT : TurbRegime {
     T sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
@@ -361,7 +387,7 @@ template <class T> struct VRawT { T sigw, dsigwdz, tlw, d0, d2; };
 // R: the regime of every active lane of the wavefront (0 neutral, 1 unstable,
 // 2 stable), or -1 for a wavefront that mixes regimes (selected per lane).
 // The arithmetic of each regime is the same in every instantiation.
-template <int R, class T>
+template <int R, bool LEAN, class T>
 __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T zeta, double log_1em3)
 {
     const float ust = t.ust, wst = t.wst, ol = t.ol, h = t.h;
@@ -369,7 +395,7 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
     const bool uns = R < 0 ? t.uns : R == 1;
     VRawT<T> v;
     // d0: neutral z/u* (corr), unstable z/L, stable -1.3 u* / h
-    v.d0 = (uns || neu ? z : kon<T>(-1.3f * ust)) / (neu ? ust : (uns ? ol : h));
+    v.d0 = qdiv<LEAN>(uns || neu ? z : kon<T>(-1.3f * ust), neu ? ust : (uns ? ol : h));
     // E: neutral e^{-2e-4 corr}, unstable e^{-5 zeta}; d2 below serves the
     // same two regimes, so a wavefront of stable particles skips both
     const bool any_nu = R < 0 ? __ballot(neu || uns) != 0ull : R != 2;
@@ -392,10 +418,10 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
         sigw = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
     }
     // d1: neutral 0.5 z / sigma_w, unstable 0.5 / sigma_w, stable 0.1 h / sigma_w
-    const T d1 = (neu ? 0.5f * z : kon<T>(uns ? 0.5f : 0.1f * h)) / sigw;
+    const T d1 = qdiv<LEAN>(neu ? 0.5f * z : kon<T>(uns ? 0.5f : 0.1f * h), sigw);
     // d2: neutral d1 / (1 + 1.5e-3 corr), unstable d1 / h
     v.d2 = kon<T>(0.0f);
-    if (any_nu) v.d2 = d1 / (neu ? 1.0f + 1.5e-3f * v.d0 : kon<T>(h));
+    if (any_nu) v.d2 = qdiv<LEAN>(d1, neu ? 1.0f + 1.5e-3f * v.d0 : kon<T>(h));
     if (neu) {
         v.dsigwdz = -2.0e-4f * sraw;
         v.tlw = v.d2;
@@ -403,8 +429,8 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
         v.dsigwdz = v.d2 * (-1.4f * ust * ust + wst * wst * (0.8f * pb - 1.8f * pa));
         const bool ca = val(z) < fabsf(ol);
         const bool cb = !ca && val(zeta) < 0.1f;
-        const T d3 = (ca ? 0.1f * z : (cb ? 0.59f * z : kon<T>(0.15f * h))) /
-                     (ca ? sigw * (0.55f - 0.38f * tabs(v.d0)) : sigw);
+        const T d3 = qdiv<LEAN>(ca ? 0.1f * z : (cb ? 0.59f * z : kon<T>(0.15f * h)),
+                                ca ? sigw * (0.55f - 0.38f * tabs(v.d0)) : sigw);
         v.tlw = (ca || cb) ? d3 : d3 * (1.0f - E);
     } else {
         v.dsigwdz = v.d0;
@@ -414,32 +440,39 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
     return v;
 }
 
-// The regime-selected evaluation: a wavefront whose active lanes share one
-// regime runs that regime's instantiation (no per-lane selects between the
-// regimes' operands), a mixed wavefront the per-lane one.
-template <class T>
-__device__ __forceinline__ VRawT<T> vertical_raw(const TurbRegime &t, T z, T zeta, double log_1em3)
-{
-    const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
-    if (n == live) return vertical_raw_r<0>(t, z, zeta, log_1em3);
-    if (u == live) return vertical_raw_r<1>(t, z, zeta, log_1em3);
-    if ((n | u) == 0ull) return vertical_raw_r<2>(t, z, zeta, log_1em3);
-    return vertical_raw_r<-1>(t, z, zeta, log_1em3);
-}
-
-// hanna_short_w: the floors of langevin_fused.wgsl:335-339
-template <class T>
-__device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, double log_1em3, T &sigw,
-                                                 T &dsigwdz, T &tlw)
+// hanna_short_w (langevin_fused.wgsl:296-344) for one regime R of every lane
+// (or R = -1, selected per lane), with the floors of :335-339
+template <int R, bool LEAN, class T>
+__device__ __forceinline__ void vertical_profile_r(const TurbRegime &t, T z_in, double log_1em3, T &sigw,
+                                                   T &dsigwdz, T &tlw)
 {
     const T z = tclamp(z_in, 0.0f, t.h);
-    const T zeta = tclamp(z / t.h, 0.0f, 1.0f);
-    const VRawT<T> v = vertical_raw(t, z, zeta, log_1em3);
+    const T zeta = tclamp(qdiv<LEAN>(z, t.h), 0.0f, 1.0f);
+    const VRawT<T> v = vertical_raw_r<R, LEAN>(t, z, zeta, log_1em3);
     tlw = tmax(v.tlw, TLW_MIN);
     dsigwdz = val(v.dsigwdz) == 0.0f ? kon<T>(1.0e-10f) : v.dsigwdz;
     sigw = (!finite_f(val(v.sigw)) || val(v.sigw) <= 0.0f) ? kon<T>(SIGMA_FLOOR) : v.sigw;
     if (!finite_f(val(tlw))) tlw = kon<T>(TLW_MIN);
     if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
+}
+
+// The regime-selected evaluation: a wavefront whose active lanes share one
+// regime and pass the range check (t.lean) runs that regime's instantiation
+// with unscaled quotients (no per-lane selects between the regimes'
+// operands); any other wavefront runs the per-lane selected one with IEEE
+// quotients.  The arithmetic of each regime is the same in every
+// instantiation, and under t.lean both quotient forms give the same values.
+template <class T>
+__device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, double log_1em3, T &sigw,
+                                                 T &dsigwdz, T &tlw)
+{
+    if (t.lean) {
+        const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
+        if (n == live) return vertical_profile_r<0, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+        if (u == live) return vertical_profile_r<1, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+        if ((n | u) == 0ull) return vertical_profile_r<2, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+    }
+    vertical_profile_r<-1, false>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
 }
 
 // The per-cell part of compute_hanna, evaluated once per PBL upload: the
@@ -458,15 +491,15 @@ __device__ __forceinline__ float4 pbl_cell_terms_dev(float4 pb)
 
 // compute_hanna (langevin_fused.wgsl:224-293) from the cell's two records,
 // for a wavefront in one regime R or a mixed one (R = -1), as vertical_raw_r
-template <int R, class T>
+template <int R, bool LEAN, class T>
 __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_1em3)
 {
     const float ust = t.ust, h = t.h;
     const bool neu = R < 0 ? t.neu : R == 0;
     const bool uns = R < 0 ? t.uns : R == 1;
     const T z = tclamp(z_in, 0.0f, h);
-    const T zeta = tclamp(z / h, 0.0f, 1.0f);
-    const VRawT<T> v = vertical_raw_r<R>(t, z, zeta, log_1em3);
+    const T zeta = tclamp(qdiv<LEAN>(z, h), 0.0f, 1.0f);
+    const VRawT<T> v = vertical_raw_r<R, LEAN>(t, z, zeta, log_1em3);
     T sigu, sigv, tlu, tlv;
     if (neu) {
         sigu = SIGMA_FLOOR + 2.0f * ust * texp(-3.0e-4f * v.d0);
@@ -480,7 +513,7 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
     }
     // unstable and stable: 0.15 h / sigma_u; neutral: T_Lu = T_Lw
     T d5 = kon<T>(0.0f);
-    if (R < 0 ? __ballot(!neu) != 0ull : R != 0) d5 = (0.15f * h) / sigu;
+    if (R < 0 ? __ballot(!neu) != 0ull : R != 0) d5 = qdiv<LEAN>(0.15f * h, sigu);
     if (neu) {
         tlu = v.d2;
         tlv = tlu;
@@ -508,8 +541,23 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
     return t;
 }
 
+// The lane's range check for unscaled quotients (fpm_div_unscaled): u*, w*
+// and h at most 2^20, the height at least 2^-40 and, in the unstable regime,
+// |L| at least 2^-20; the launch adds dt in [2^-20, 2^20] and z_min in
+// [2^-40, 1) (so that every sub-step height lies in [z_min, h]).  Then every
+// numerator and denominator of compute_hanna, hanna_short_w and the two
+// Ornstein-Uhlenbeck updates lies in [2^-47, 2^41] and every quotient in
+// [2^-90, 2^40]: sigma_w, sigma_u >= 0.01 (the 0.01 terms, the unstable
+// sigma_w^2 >= 0.4 u*^2), all sigmas <= 2^34, 0.55 - 0.38 z/|L| > 0.17 where
+// it divides, 1 + 1.5e-3 z/u* <= 2^24, T_L in [10, 2^27].
+__device__ __forceinline__ bool lean_lane(float ust, float wst, float h, float z, bool uns, float ol)
+{
+    return fmaxf(fmaxf(ust, wst), h) <= 0x1p20f && z >= 0x1p-40f && (!uns || fabsf(ol) >= 0x1p-20f);
+}
+
+// launch_lean: the launch's part of the range check (StepArgs::lean_div)
 template <class T>
-__device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, double log_1em3)
+__device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, double log_1em3, bool launch_lean = false)
 {
     TurbT<T> t;
     t.ust = fmaxf(pb.x, USTAR_MIN);
@@ -520,11 +568,14 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
     const uint32_t regime = __float_as_uint(pc.z);
     t.neu = regime == 0u;
     t.uns = regime == 1u;
-    const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
-    if (n == live) return hanna_full_r<0>(t, z_in, log_1em3);
-    if (u == live) return hanna_full_r<1>(t, z_in, log_1em3);
-    if ((n | u) == 0ull) return hanna_full_r<2>(t, z_in, log_1em3);
-    return hanna_full_r<-1>(t, z_in, log_1em3);
+    t.lean = launch_lean && __ballot(!lean_lane(t.ust, t.wst, t.h, val(z_in), t.uns, t.ol)) == 0ull;
+    if (t.lean) {
+        const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
+        if (n == live) return hanna_full_r<0, true>(t, z_in, log_1em3);
+        if (u == live) return hanna_full_r<1, true>(t, z_in, log_1em3);
+        if ((n | u) == 0ull) return hanna_full_r<2, true>(t, z_in, log_1em3);
+    }
+    return hanna_full_r<-1, false>(t, z_in, log_1em3);
 }
 
 // ---------------------------------------------------------------- Langevin
@@ -541,10 +592,10 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
 // floor).  The reference's clamps of both to [0, inf) and its T_L == 0 case
 // are then the identity and false, and are not evaluated.
 
-template <class T>
+template <bool LEAN = false, class T>
 __device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, float eta)
 {
-    const T r = dt / tl;
+    const T r = qdiv<LEAN>(dt, tl);
     const bool small = val(r) < 0.5f;
     // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
     T e = kon<T>(0.0f);
@@ -554,11 +605,11 @@ __device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, floa
     return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
 
-template <class T>
+template <bool LEAN = false, class T>
 __device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt, float rho, float eta)
 {
     const T drift = sw * (dsigwdz + rho * sw);
-    const T r = dt / tl;
+    const T r = qdiv<LEAN>(dt, tl);
     const bool small = val(r) < 0.5f;
     T e = kon<T>(0.0f);
     if (__ballot(!small) != 0ull) e = texp(-r);

@@ -79,7 +79,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         if (live) {
 
             // ---- advection (advection_dual_wind.wgsl:254-303)
-            const float nxm = (float)(a.nx - 1u), nym = (float)(a.ny - 1u);
+            const float nxm = a.nxm, nym = a.nym;
             const float x0 = (float)cx + px, y0 = (float)cy + py, z0 = pz;
             const Wind s0 = sample_wind(a, lh, alpha, x0, y0, z0);
             const float u0 = s0.u + tu, v0 = s0.v + tv, w0 = s0.w;
@@ -92,8 +92,8 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             const float x1 = clampq(x0 + (cu * a.x_scale) * a.dt_adv, 0.0f, nxm);
             const float y1 = clampq(y0 + (cv * a.y_scale) * a.dt_adv, 0.0f, nym);
             pz = clampq(z0 + (cw * a.z_scale) * a.dt_adv, 0.0f, a.zmax);
-            split_cell(x1, a.nx, cx, px);
-            split_cell(y1, a.ny, cy, py);
+            split_cell(x1, a.nx, nxm, cx, px);
+            split_cell(y1, a.ny, nym, cy, py);
 
             // ---- PBL lookup + Hanna (langevin_fused.wgsl:414-427)
             const float hx = (float)cx + px, hy = (float)cy + py;
@@ -114,23 +114,30 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             gauss_pair(b0.x, b0.y, eu, ev);
             gauss_pair(b0.z, b0.w, ew0, ew1);
             gauss_pair(b1.x, b1.y, ew2, ew3);
-            const Turb t = hanna_full(pbl0, pbl1, pz, a.log_1em3);
+            const Turb t = hanna_full(pbl0, pbl1, pz, a.log_1em3, a.lean_div != 0u);
 
             // ---- horizontal OU over the full step
-            tu = ou_horizontal(tu, t.sigu, t.tlu, a.dt, eu);
-            tv = ou_horizontal(tv, t.sigv, t.tlv, a.dt, ev);
+            if (t.lean) {
+                tu = ou_horizontal<true>(tu, t.sigu, t.tlu, a.dt, eu);
+                tv = ou_horizontal<true>(tv, t.sigv, t.tlv, a.dt, ev);
+            } else {
+                tu = ou_horizontal(tu, t.sigu, t.tlu, a.dt, eu);
+                tv = ou_horizontal(tv, t.sigv, t.tlv, a.dt, ev);
+            }
 
             // ---- vertical OU with sub-steps and reflection
             if (nsub == 0u) {
-                tw = ou_vertical(tw, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0);
+                tw = t.lean ? ou_vertical<true>(tw, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0)
+                            : ou_vertical(tw, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0);
             } else {
-                const float dts = a.dt / (float)nsub;
+                const float dts = a.dt_sub;  // a.dt / n_substeps
                 const float zmin = a.min_height;
                 float sigw = t.sigw, dsw = t.dsigwdz, tlw = t.tlw;
 #pragma unroll
                 for (uint32_t sub = 0; sub < (NSUB > 0 ? (uint32_t)NSUB : nsub); ++sub) {
                     const float eta = sub == 0u ? ew0 : (sub == 1u ? ew1 : (sub == 2u ? ew2 : ew3));
-                    tw = ou_vertical(tw, sigw, dsw, tlw, dts, a.rho_grad, eta);
+                    if (t.lean) tw = ou_vertical<true>(tw, sigw, dsw, tlw, dts, a.rho_grad, eta);
+                    else tw = ou_vertical(tw, sigw, dsw, tlw, dts, a.rho_grad, eta);
                     pz = pz + tw * dts;
                     reflect(pz, tw, cbt, zmin, t.h);
                     // hanna_short_w at the new height (langevin_fused.wgsl:502-513)
@@ -140,8 +147,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
 
             // ---- dry deposition (dry_deposition.wgsl:70-106)
             if (a.dry_mode != FP_DRY_OFF) {
-                const float href = fmaxf(a.dry_href, 0.1f);
-                if (!(pz >= 2.0f * href)) {
+                if (!(pz >= a.dry_zcut)) {
                     float sv = a.dry_sv;
                     if (a.dry_mode == FP_DRY_PER_SLOT) sv = dry_survival(a.dry_vd_slots[at], a.dt, a.dry_href);
 #pragma unroll

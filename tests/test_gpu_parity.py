@@ -113,6 +113,54 @@ def test_step_matches_oracle_bitwise(fp, orc, small, case):
     assert_particles_equal(got, want, f"step {case}")
 
 
+@pytest.mark.parametrize("min_height", [0.01, 2.0 ** -40, 0.0])
+def test_step_at_the_quotient_range_check_matches_oracle(fp, orc, workloads, min_height):
+    """Wavefronts whose cells and heights pass the range check evaluate the
+    turbulence quotients without range scaling; the others (and every
+    wavefront when z_min is below 2^-40) take the IEEE quotients.  PBL cells
+    at, just inside and just outside each bound of the check (u*, w*, h at
+    2^20, |L| at 2^-20) and beyond it, and particle heights at 0, 2^-41,
+    2^-40 and ordinary values, mixed within wavefronts: bit for bit."""
+    say(f"step parity at the quotient range check: z_min={min_height}")
+    w = workloads.small_random(8192, shape=(16, 12, 24), seed=5)
+    rng = np.random.default_rng(17)
+    n = w.pbl.ustar.size
+    edge = np.float32(2.0 ** 20)
+    above = np.nextafter(edge, np.float32(np.inf))
+    pick = lambda vals, p: rng.choice(np.array(vals, np.float32), n, p=p)
+    ustar = np.where(rng.uniform(size=n) < 0.6, w.pbl.ustar, pick([edge, above, 1e25], [0.4, 0.4, 0.2]))
+    wstar = np.where(rng.uniform(size=n) < 0.7, w.pbl.wstar, pick([edge, above, 3e30], [0.4, 0.4, 0.2]))
+    hmix = np.where(rng.uniform(size=n) < 0.6, w.pbl.hmix, pick([edge, above, 1e30], [0.4, 0.4, 0.2]))
+    oli = np.where(rng.uniform(size=n) < 0.6, w.pbl.oli,
+                   pick([-edge, -above, -1e20, -edge * 0.5], [0.3, 0.3, 0.2, 0.2]))
+    parts = w.particles.copy()
+    z = parts["pos_z"]
+    sel = rng.uniform(size=z.size)
+    z[sel < 0.05] = 0.0
+    z[(sel >= 0.05) & (sel < 0.1)] = np.float32(2.0 ** -41)
+    z[(sel >= 0.1) & (sel < 0.15)] = np.float32(2.0 ** -40)
+    z[(sel >= 0.15) & (sel < 0.2)] = np.float32(1e-30)
+    met = met_dict(w.met)
+    pbl = dict(shape=w.pbl.shape, ustar=ustar.astype(np.float32), wstar=wstar.astype(np.float32),
+               hmix=hmix.astype(np.float32), oli=oli.astype(np.float32))
+    want = parts.copy()
+    with fp.GpuTimestep(len(parts), w.met.shape, n_species=4) as ctx:
+        u0, v0, w0 = w.met.levels[0]
+        u1, v1, w1 = w.met.levels[1]
+        ctx.upload_met_bracket(u0, v0, w0, u1, v1, w1, w.met.level_heights)
+        ctx.upload_pbl(pbl["ustar"], pbl["wstar"], pbl["hmix"], pbl["oli"])
+        ctx.upload_store(parts)
+        counter = (7, 0, 0, 0)
+        for _ in range(3):
+            cfg = base_cfg(fp, w, counter=counter, min_height_m=min_height)
+            nxt = ctx.step(cfg)
+            onxt = orc.step(want, met, pbl, oracle_params(w, cfg))
+            assert list(nxt) == [int(v) for v in onxt]
+            counter = nxt
+        got = ctx.download_particles()
+    assert_particles_equal(got, want, f"range check z_min={min_height}")
+
+
 @pytest.mark.parametrize("case", ["production", "three_substeps"])
 @pytest.mark.parametrize("ppt", [1, 2, 4])
 @pytest.mark.parametrize("block", [64, 128, 256, 512, 1024])
