@@ -451,9 +451,14 @@ __device__ __forceinline__ void vertical_profile_r(const TurbRegime &t, T z_in, 
     const VRawT<T> v = vertical_raw_r<R, LEAN>(t, z, zeta, log_1em3);
     tlw = tmax(v.tlw, TLW_MIN);
     dsigwdz = val(v.dsigwdz) == 0.0f ? kon<T>(1.0e-10f) : v.dsigwdz;
-    sigw = (!finite_f(val(v.sigw)) || val(v.sigw) <= 0.0f) ? kon<T>(SIGMA_FLOOR) : v.sigw;
-    if (!finite_f(val(tlw))) tlw = kon<T>(TLW_MIN);
-    if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
+    sigw = v.sigw;
+    // Under LEAN sigma_w is finite and >= 0.01, T_L_w and dsigma_w/dz finite
+    // (lean_lane), so these floors change nothing
+    if (!LEAN) {
+        if (!finite_f(val(v.sigw)) || val(v.sigw) <= 0.0f) sigw = kon<T>(SIGMA_FLOOR);
+        if (!finite_f(val(tlw))) tlw = kon<T>(TLW_MIN);
+        if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
+    }
 }
 
 // The regime-selected evaluation: a wavefront whose active lanes share one
@@ -529,13 +534,18 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
     tlv = tmax(tlv, TLV_MIN);
     tlw = tmax(tlw, TLW_MIN);
     if (val(dsigwdz) == 0.0f) dsigwdz = kon<T>(1.0e-10f);
-    if (!finite_f(val(sigu)) || val(sigu) <= 0.0f) sigu = kon<T>(SIGMA_FLOOR);
-    if (!finite_f(val(sigv)) || val(sigv) <= 0.0f) sigv = kon<T>(SIGMA_FLOOR);
-    if (!finite_f(val(sigw)) || val(sigw) <= 0.0f) sigw = kon<T>(SIGMA_FLOOR);
-    if (!finite_f(val(tlu))) tlu = kon<T>(TLU_MIN);
-    if (!finite_f(val(tlv))) tlv = kon<T>(TLV_MIN);
-    if (!finite_f(val(tlw))) tlw = kon<T>(TLW_MIN);
-    if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
+    // Under LEAN every sigma is finite and >= 0.01 (the cell's unstable
+    // sigma_u too: 0.01 + u* (12 + 0.5 h/|L|)^(1/3) <= 2^34) and every T_L and
+    // dsigma_w/dz finite (lean_lane), so these floors change nothing
+    if (!LEAN) {
+        if (!finite_f(val(sigu)) || val(sigu) <= 0.0f) sigu = kon<T>(SIGMA_FLOOR);
+        if (!finite_f(val(sigv)) || val(sigv) <= 0.0f) sigv = kon<T>(SIGMA_FLOOR);
+        if (!finite_f(val(sigw)) || val(sigw) <= 0.0f) sigw = kon<T>(SIGMA_FLOOR);
+        if (!finite_f(val(tlu))) tlu = kon<T>(TLU_MIN);
+        if (!finite_f(val(tlv))) tlv = kon<T>(TLV_MIN);
+        if (!finite_f(val(tlw))) tlw = kon<T>(TLW_MIN);
+        if (!finite_f(val(dsigwdz))) dsigwdz = kon<T>(1.0e-10f);
+    }
     t.sigu = sigu; t.sigv = sigv; t.sigw = sigw; t.dsigwdz = dsigwdz;
     t.tlu = tlu; t.tlv = tlv; t.tlw = tlw;
     return t;
@@ -592,31 +602,61 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
 // floor).  The reference's clamps of both to [0, inf) and its T_L == 0 case
 // are then the identity and false, and are not evaluated.
 
+// BR: the branch of every lane of the wavefront (0: r < 0.5, 1: r >= 0.5) or
+// -1 for a wavefront with both (selected per lane).  Under LEAN, r > 0 (dt
+// >= 2^-20, T_L >= 10) and e^{-r} lies in [0, 1], so the reference's clamps
+// of 2 r and 1 - e^{-2r} to [0, inf) are the identity.
+template <int BR, bool LEAN, class T>
+__device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta)
+{
+    const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
+    // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
+    T e = kon<T>(0.0f);
+    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
+    const T arg = small ? 2.0f * r : 1.0f - e * e;
+    const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
+    const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
+    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
+}
+
+template <int BR, bool LEAN, class T>
+__device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, float dt, float rho, float eta)
+{
+    const T drift = sw * (dsigwdz + rho * sw);
+    const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
+    T e = kon<T>(0.0f);
+    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
+    const T arg = small ? 2.0f * r : 1.0f - e * e;
+    const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
+    const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
+                 (small ? dt * drift : tl * (1.0f - e) * drift);
+    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
+}
+
+// A wavefront whose lanes all take one branch runs that branch alone (under
+// LEAN; the arithmetic of each branch is the same as in the per-lane form).
 template <bool LEAN = false, class T>
 __device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, float eta)
 {
     const T r = qdiv<LEAN>(dt, tl);
-    const bool small = val(r) < 0.5f;
-    // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
-    T e = kon<T>(0.0f);
-    if (__ballot(!small) != 0ull) e = texp(-r);
-    const T root = tsqrt_finite(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
-    const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
-    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
+    if (LEAN) {
+        const uint64_t sm = __ballot(val(r) < 0.5f), live = __ballot(1);
+        if (sm == live) return ou_horizontal_b<0, LEAN>(prev, sigma, r, eta);
+        if (sm == 0ull) return ou_horizontal_b<1, LEAN>(prev, sigma, r, eta);
+    }
+    return ou_horizontal_b<-1, LEAN>(prev, sigma, r, eta);
 }
 
 template <bool LEAN = false, class T>
 __device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt, float rho, float eta)
 {
-    const T drift = sw * (dsigwdz + rho * sw);
     const T r = qdiv<LEAN>(dt, tl);
-    const bool small = val(r) < 0.5f;
-    T e = kon<T>(0.0f);
-    if (__ballot(!small) != 0ull) e = texp(-r);
-    const T root = tsqrt_finite(small ? tmax(2.0f * r, 0.0f) : tmax(1.0f - e * e, 0.0f));
-    const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
-                 (small ? dt * drift : tl * (1.0f - e) * drift);
-    return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
+    if (LEAN) {
+        const uint64_t sm = __ballot(val(r) < 0.5f), live = __ballot(1);
+        if (sm == live) return ou_vertical_b<0, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
+        if (sm == 0ull) return ou_vertical_b<1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
+    }
+    return ou_vertical_b<-1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
 }
 
 // PBL reflection (langevin_fused.wgsl:481-500)
