@@ -127,16 +127,6 @@ __device__ __forceinline__ WindGrad sample_wind_grad(const StepArgs &a, const fl
     return g;
 }
 
-// d(new)/d(prev) of the Langevin update (the retained fraction of the old
-// velocity): 1 - r or e^{-r} with r = dt / T_L; 0 when the update is zeroed.
-__device__ __forceinline__ float ou_coef(float tl_in, float dt)
-{
-    const float tl = pos_or_zero(tl_in);
-    if (tl == 0.0f) return 0.0f;
-    const float r = dt / tl;
-    return r < 0.5f ? 1.0f - r : fpm_expf(-r);
-}
-
 // Wave-level sum of the (u, v) perturbation adjoints of one wind sample over
 // lanes that share its horizontal cell, then 8 f64 atomics per distinct cell.
 __device__ __forceinline__ void scatter_wind_adjoint(const AdjArgs &g, bool valid, const WindGrad &s, float lu,
@@ -241,16 +231,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         gauss_pair(b0.z, b0.w, ew0, ew1);
         gauss_pair(b1.x, b1.y, ew2, ew3);
 
-        const Dz tun = ou_horizontal(Dz{tu, 0.0f}, t.sigu, t.tlu, a.dt, eu);
-        const Dz tvn = ou_horizontal(Dz{tv, 0.0f}, t.sigv, t.tlv, a.dt, ev);
-        const float cu_prev = ou_coef(t.tlu.v, a.dt), cv_prev = ou_coef(t.tlv.v, a.dt);
+        float cu_prev, cv_prev;  // d u'/d u, d v'/d v (the retained fractions)
+        Dz tun, tvn;
+        if (t.lean) {
+            tun = ou_horizontal<true>(Dz{tu, 0.0f}, t.sigu, t.tlu, a.dt, eu, &cu_prev);
+            tvn = ou_horizontal<true>(Dz{tv, 0.0f}, t.sigv, t.tlv, a.dt, ev, &cv_prev);
+        } else {
+            tun = ou_horizontal(Dz{tu, 0.0f}, t.sigu, t.tlu, a.dt, eu, &cu_prev);
+            tvn = ou_horizontal(Dz{tv, 0.0f}, t.sigv, t.tlv, a.dt, ev, &cv_prev);
+        }
 
         float A[4], G[4], S[4];
         bool K[4];
         float zf = z1;
         if (nsub == 0u) {
-            const Dz twn = ou_vertical(Dz{tw, 0.0f}, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0);
-            A[0] = ou_coef(t.tlw.v, a.dt);
+            const Dz twn = t.lean ? ou_vertical<true>(Dz{tw, 0.0f}, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0, &A[0])
+                                  : ou_vertical(Dz{tw, 0.0f}, t.sigw, t.dsigwdz, t.tlw, a.dt, a.rho_grad, ew0, &A[0]);
             G[0] = twn.d;
         } else {
             const float dts = a.dt_sub;  // a.dt / n_substeps
@@ -261,8 +257,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
             for (uint32_t sub = 0; sub < (NSUB > 0 ? (uint32_t)NSUB : 4u); ++sub) {
                 if (NSUB < 0 && sub >= nsub) break;
                 const float eta = sub == 0u ? ew0 : (sub == 1u ? ew1 : (sub == 2u ? ew2 : ew3));
-                const Dz wn = ou_vertical(Dz{w, 0.0f}, sigw, dsw, tlw, dts, a.rho_grad, eta);
-                A[sub] = ou_coef(tlw.v, dts);
+                Dz wn;
+                if (t.lean) wn = ou_vertical<true>(Dz{w, 0.0f}, sigw, dsw, tlw, dts, a.rho_grad, eta, &A[sub]);
+                else wn = ou_vertical(Dz{w, 0.0f}, sigw, dsw, tlw, dts, a.rho_grad, eta, &A[sub]);
                 G[sub] = wn.d;
                 float zz = z + wn.v * dts;
                 float sg = 1.0f;

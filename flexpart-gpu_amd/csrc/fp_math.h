@@ -85,6 +85,28 @@ FPM_FN float fpm_clampf(float x, float lo, float hi)
 #endif
 }
 
+/* fpm_div_unscaled that also returns the Newton-refined reciprocal of d it
+ * multiplies by (about one unit in the last place of 1/d).  The quotient is
+ * the same. */
+FPM_FN float fpm_div_unscaled_rcp(float n, float d, float *rcp)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float nd = -d;
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e0 = __builtin_fmaf(nd, r, 1.0f);
+    r = __builtin_fmaf(e0, r, r);
+    float q = n * r;
+    const float e1 = __builtin_fmaf(nd, q, n);
+    q = __builtin_fmaf(e1, r, q);
+    const float e2 = __builtin_fmaf(nd, q, n);
+    *rcp = r;
+    return __builtin_fmaf(e2, r, q);
+#else
+    *rcp = 1.0f / d;
+    return n / d;
+#endif
+}
+
 /* e^x, single precision.  Special cases are selected at the end rather than
  * branched on, so a wavefront never splits inside the function. */
 FPM_FN float fpm_expf(float x)

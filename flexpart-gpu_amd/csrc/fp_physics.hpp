@@ -43,12 +43,14 @@ __device__ __forceinline__ float mix(float a, float b, float t, float u) { retur
 // checked exhaustively on gfx950 by an exact binary64 test of the rounding
 // (fp_math_sweep).  Below 2^-96 the residual underflows and the result can be
 // off by one unit in the last place.
-__device__ __forceinline__ float sqrt_in_range(float x)
+__device__ __forceinline__ float sqrt_in_range(float x, float *half_rsqrt = nullptr)
 {
     const float s0 = __builtin_amdgcn_sqrtf(x);
     const float y = __builtin_amdgcn_rsqf(x);
     const float d = __builtin_fmaf(-s0, s0, x);
-    return __builtin_fmaf(d, 0.5f * y, s0);
+    const float hy = 0.5f * y;
+    if (half_rsqrt) *half_rsqrt = hy;  // about 1 / (2 sqrt(x))
+    return __builtin_fmaf(d, hy, s0);
 }
 
 // Correctly rounded square root of any x: a wavefront with a lane outside
@@ -266,17 +268,29 @@ __device__ __forceinline__ Dz texp(Dz x)
     return Dz{e, e * x.d};
 }
 __device__ __forceinline__ float tsqrt(float x) { return sqrt_cr(x); }
+// (the derivative 1 / (2 sqrt(x)) from the root's own reciprocal square root
+// where the short sequence runs)
 __device__ __forceinline__ Dz tsqrt(Dz x)
 {
-    const float r = sqrt_cr(x.v);
-    return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
+    if (__builtin_expect(__ballot(!(x.v >= 0x1p-96f && x.v <= F32_MAX)) != 0ull, 0)) {
+        const float r = __builtin_sqrtf(x.v);
+        return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
+    }
+    float hy;
+    const float r = sqrt_in_range(x.v, &hy);
+    return Dz{r, x.d * hy};
 }
 // (of an argument known not to be +infinity)
 __device__ __forceinline__ float tsqrt_finite(float x) { return sqrt_cr_finite(x); }
 __device__ __forceinline__ Dz tsqrt_finite(Dz x)
 {
-    const float r = sqrt_cr_finite(x.v);
-    return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
+    if (__builtin_expect(__ballot(!(x.v >= 0x1p-96f)) != 0ull, 0)) {
+        const float r = __builtin_sqrtf(x.v);
+        return Dz{r, r > 0.0f ? x.d / (2.0f * r) : 0.0f};
+    }
+    float hy;
+    const float r = sqrt_in_range(x.v, &hy);
+    return Dz{r, x.d * hy};
 }
 __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 __device__ __forceinline__ Dz tabs(Dz x) { return Dz{fabsf(x.v), x.v < 0.0f ? -x.d : x.d}; }
@@ -302,9 +316,30 @@ template <bool LEAN> __device__ __forceinline__ float qdiv(float n, float d)
 {
     return LEAN ? fpm_div_unscaled(n, d) : n / d;
 }
-template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, Dz d) { return n / d; }
-template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, float d) { return n / d; }
-template <bool LEAN> __device__ __forceinline__ Dz qdiv(float n, Dz d) { return n / d; }
+// (Under LEAN the height derivative multiplies by the refined reciprocal of
+// the denominator instead of dividing by it: the value is the same as the
+// forward step's, the derivative within a unit or two in the last place.)
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, Dz d)
+{
+    if (!LEAN) return n / d;
+    float r;
+    const float q = fpm_div_unscaled_rcp(n.v, d.v, &r);
+    return Dz{q, (n.d - q * d.d) * r};
+}
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(Dz n, float d)
+{
+    if (!LEAN) return n / d;
+    float r;
+    const float q = fpm_div_unscaled_rcp(n.v, d, &r);
+    return Dz{q, n.d * r};
+}
+template <bool LEAN> __device__ __forceinline__ Dz qdiv(float n, Dz d)
+{
+    if (!LEAN) return n / d;
+    float r;
+    const float q = fpm_div_unscaled_rcp(n, d.v, &r);
+    return Dz{q, -q * d.d * r};
+}
 
 // The binary64 log/exp tables of fp_math.h, copied into LDS by the kernels
 // that evaluate the Hanna powers (load_math_tab, before the kernel's first
@@ -607,7 +642,7 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
 // >= 2^-20, T_L >= 10) and e^{-r} lies in [0, 1], so the reference's clamps
 // of 2 r and 1 - e^{-2r} to [0, inf) are the identity.
 template <int BR, bool LEAN, class T>
-__device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta)
+__device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta, float *coef)
 {
     const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
     // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
@@ -615,12 +650,15 @@ __device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta)
     if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
     const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
-    const T nx = (small ? 1.0f - r : e) * prev + eta * sigma * root;
+    const T c = small ? 1.0f - r : e;
+    if (coef) *coef = val(c);
+    const T nx = c * prev + eta * sigma * root;
     return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
 
 template <int BR, bool LEAN, class T>
-__device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, float dt, float rho, float eta)
+__device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, float dt, float rho, float eta,
+                                           float *coef)
 {
     const T drift = sw * (dsigwdz + rho * sw);
     const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
@@ -628,35 +666,39 @@ __device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, f
     if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
     const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
-    const T nx = (small ? 1.0f - r : e) * prev + eta * sw * root +
-                 (small ? dt * drift : tl * (1.0f - e) * drift);
+    const T c = small ? 1.0f - r : e;
+    if (coef) *coef = val(c);
+    const T nx = c * prev + eta * sw * root + (small ? dt * drift : tl * (1.0f - e) * drift);
     return !finite_f(val(nx)) ? kon<T>(0.0f) : nx;
 }
 
 // A wavefront whose lanes all take one branch runs that branch alone (under
 // LEAN; the arithmetic of each branch is the same as in the per-lane form).
+// coef, if given, receives the retained fraction of the old velocity, 1 - r
+// or e^{-r} (d new / d prev at fixed sigma and T_L).
 template <bool LEAN = false, class T>
-__device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, float eta)
+__device__ __forceinline__ T ou_horizontal(T prev, T sigma, T tl, float dt, float eta, float *coef = nullptr)
 {
     const T r = qdiv<LEAN>(dt, tl);
     if (LEAN) {
         const uint64_t sm = __ballot(val(r) < 0.5f), live = __ballot(1);
-        if (sm == live) return ou_horizontal_b<0, LEAN>(prev, sigma, r, eta);
-        if (sm == 0ull) return ou_horizontal_b<1, LEAN>(prev, sigma, r, eta);
+        if (sm == live) return ou_horizontal_b<0, LEAN>(prev, sigma, r, eta, coef);
+        if (sm == 0ull) return ou_horizontal_b<1, LEAN>(prev, sigma, r, eta, coef);
     }
-    return ou_horizontal_b<-1, LEAN>(prev, sigma, r, eta);
+    return ou_horizontal_b<-1, LEAN>(prev, sigma, r, eta, coef);
 }
 
 template <bool LEAN = false, class T>
-__device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt, float rho, float eta)
+__device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt, float rho, float eta,
+                                         float *coef = nullptr)
 {
     const T r = qdiv<LEAN>(dt, tl);
     if (LEAN) {
         const uint64_t sm = __ballot(val(r) < 0.5f), live = __ballot(1);
-        if (sm == live) return ou_vertical_b<0, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
-        if (sm == 0ull) return ou_vertical_b<1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
+        if (sm == live) return ou_vertical_b<0, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta, coef);
+        if (sm == 0ull) return ou_vertical_b<1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta, coef);
     }
-    return ou_vertical_b<-1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta);
+    return ou_vertical_b<-1, LEAN>(prev, sw, dsigwdz, tl, r, dt, rho, eta, coef);
 }
 
 // PBL reflection (langevin_fused.wgsl:481-500)
