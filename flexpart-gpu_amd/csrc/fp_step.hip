@@ -48,14 +48,15 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
     // normals (known at compile time for NSUB >= 0, so both blocks interleave)
     const bool two_blocks = NSUB >= 0 ? NSUB > 2 : a.blocks_per_particle > 1u;
     for (uint32_t r = 0; r < (ROWS ? ROWS : a.ppt); ++r) {
-        const uint64_t i = first + (uint64_t)r * BLOCK + threadIdx.x;
+        // slot indices fit 32 bits (slot ranges are limited to 2^30 slots)
+        const uint32_t i = (uint32_t)first + r * BLOCK + threadIdx.x;
         const bool inb = i < a.n;
         // the slot the state is read from
-        uint64_t g = i;
+        uint32_t g = i;
         if (GATHER != kGatherNone && inb && a.gather_map) g = a.gather_map[i];
         const Soa sl = GATHER != kGatherNone ? a.src : s;
         // the slot the particle occupies during the step
-        const uint64_t at = GATHER == kGatherFromSlot ? g : i;
+        const uint32_t at = GATHER == kGatherFromSlot ? g : i;
         // The slot's whole hot state is requested at once (44 B, coalesced);
         // slots whose flags say inactive are not used further.
         uint32_t flags = 0;
@@ -106,7 +107,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             // Box-Muller normals (they depend on the slot index only; drawn here,
             // while the PBL records load, so that they are not live across the
             // wind samples)
-            const uint64_t cbase = at * a.blocks_per_particle;
+            const uint64_t cbase = (uint64_t)at * a.blocks_per_particle;
             const U4 b0 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase), a.key0, a.key1);
             U4 b1 = U4{0u, 0u, 0u, 0u};
             if (two_blocks) b1 = philox10(ctr_add(ctr.x, ctr.y, ctr.z, ctr.w, cbase + 1u), a.key0, a.key1);
@@ -213,11 +214,14 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 // production value is 4, timeloop.rs:1111); NSUB < 0 reads it at run time.
 // ROWS = 1 specialises one particle per thread.  The kernel is bound by
 // vector-ALU issue and the latency of its gathers, so resident wavefronts
-// matter: the single-row kernel needs 84 VGPRs and runs 5 wavefronts per SIMD
-// (6 would spill); with a run-time row loop the compiler keeps loop-invariant
-// values live across the rows and needs 128 VGPRs (4 wavefronts).
+// matter: the single-row kernel fits 77 VGPRs (6 wavefronts per SIMD) and,
+// capped at 72, runs 7 with 16 bytes of spills, which the in-place step gains
+// from (1-3 %) and the gathered steps lose to; with a run-time row loop the
+// compiler keeps loop-invariant values live across the rows and needs 128
+// VGPRs (4 wavefronts).
 template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4))) step_kernel(StepArgs a)
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(ROWS == 1 ? (GATHER == kGatherNone ? 7 : 5) : 4))) step_kernel(StepArgs a)
 {
     __shared__ float lh[kLevelTableMax];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
