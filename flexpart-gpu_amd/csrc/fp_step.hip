@@ -216,12 +216,19 @@ __device__ __forceinline__ void load_tables(const StepArgs &a, float *lh, float 
 // vector-ALU issue and the latency of its gathers, so resident wavefronts
 // matter: the single-row kernel fits 77 VGPRs (6 wavefronts per SIMD) and,
 // capped at 72, runs 7 with 16 bytes of spills, which the in-place step gains
-// from (1-3 %) and the gathered steps lose to; with a run-time row loop the
-// compiler keeps loop-invariant values live across the rows and needs 128
+// from (1-3 %) and the gathered steps lose to.  Only blocks of up to 256
+// threads (at most one wavefront per SIMD each) can fill 7 wavefronts per
+// SIMD; larger blocks keep the uncapped allocation.  With a run-time row loop
+// the compiler keeps loop-invariant values live across the rows and needs 128
 // VGPRs (4 wavefronts).
+template <uint32_t BLOCK, int NSUB, int GATHER>
+constexpr int step_waves_per_eu(uint32_t rows)
+{
+    return rows != 1u ? 4 : (GATHER == kGatherNone && BLOCK <= 256u ? 7 : 5);
+}
 template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
 __global__ void __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(ROWS == 1 ? (GATHER == kGatherNone ? 7 : 5) : 4))) step_kernel(StepArgs a)
+__attribute__((amdgpu_waves_per_eu(step_waves_per_eu<BLOCK, NSUB, GATHER>(ROWS)))) step_kernel(StepArgs a)
 {
     __shared__ float lh[kLevelTableMax];
     __shared__ float oh[FP_MAX_OUTPUT_LEVELS];
