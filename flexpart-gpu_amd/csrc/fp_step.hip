@@ -235,7 +235,13 @@ __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<BLOCK, NSUB, GATHER>(ROWS))
     __shared__ GridLds gt;
     if (a.grid_fused) grid_lds_clear<BLOCK>(gt);
     load_tables<BLOCK>(a, lh, oh);
-    const uint64_t first = (uint64_t)xcd_chunk(blockIdx.x, gridDim.x) * BLOCK * a.ppt;
+    // the in-place step at 512 threads and more on 4 regions per XCD (measured
+    // 1-2 % faster than 1 or 2 regions, 8 slower; at 256 threads 4 regions were
+    // 15 % slower on the later steps); the other shapes and the gathered steps
+    // keep one
+    const uint32_t chunk = GATHER == kGatherNone && BLOCK >= 512u ? xcd_chunk_rounds(blockIdx.x, gridDim.x, 4u)
+                                                                  : xcd_chunk(blockIdx.x, gridDim.x);
+    const uint64_t first = (uint64_t)chunk * BLOCK * a.ppt;
     step_chunk<BLOCK, NSUB, GATHER, ROWS>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
     if (a.grid_fused) grid_lds_flush<BLOCK>(gt, a.grid);
 }
