@@ -652,7 +652,8 @@ class GpuTimestep:
         return out
 
     # -- device arithmetic probe
-    PROBES = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "pow": 4, "sqrt": 5, "div": 6, "log_unit": 7}
+    PROBES = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "pow": 4, "sqrt": 5, "div": 6, "log_unit": 7,
+              "level": 8}
 
     def math_probe(self, fn: str, x, y=None) -> np.ndarray:
         xs = _f32(x)

@@ -69,7 +69,7 @@ struct WindGrad {
     float tx, ty;
 };
 
-__device__ __forceinline__ void level_grad(float z, const float *lh, uint32_t nz, bool heights, float &lev,
+__device__ __forceinline__ void level_grad(float z, const float *lh, uint32_t nz, uint32_t heights, float &lev,
                                            float &dlev)
 {
     lev = level_of(z, lh, nz, heights);
@@ -95,7 +95,7 @@ __device__ __forceinline__ WindGrad sample_wind_grad(const StepArgs &a, const fl
     const Axis ax = axis_of(x, a.nx, a.nxm);
     const Axis ay = axis_of(y, a.ny, a.nym);
     float lev, dlev;
-    level_grad(z, lh, a.nz, a.has_heights != 0, lev, dlev);
+    level_grad(z, lh, a.nz, a.has_heights, lev, dlev);
     const Axis az = a.has_heights ? axis_of(lev, a.nz, a.nzm) : axis_of<true>(lev, a.nz, a.nzm);
     const uint32_t ny = a.ny, nz = a.nz;
     const float ux = 1.0f - ax.t, uy = 1.0f - ay.t, uz = 1.0f - az.t, ua = 1.0f - alpha;
@@ -723,7 +723,8 @@ struct fp_adj {
     float4 *met_packed = nullptr;  // (n_met_levels - 1) brackets x npts x 2
     float *level_heights = nullptr;
     std::vector<float> lh_host;
-    bool has_heights = false, met_ready = false;
+    uint32_t has_heights = fp::kLevelIndex;
+    bool met_ready = false;
     float zmax = 0.0f;
     double level_seconds = 0.0;
 
@@ -850,7 +851,7 @@ fp::StepArgs step_args(const fp_adj *e, uint32_t k, const fp::Soa &s)
     a.ny = e->d.met_ny;
     a.nz = e->d.met_nz;
     a.level_heights = e->level_heights;
-    a.has_heights = e->has_heights ? 1u : 0u;
+    a.has_heights = e->has_heights;
     a.zmax = e->zmax;
     a.pbl = e->pbl;
     a.precip = e->precip;
@@ -1026,8 +1027,7 @@ int fp_adj_met_upload(fp_adj *e, const float *u, const float *v, const float *w,
     const bool metres = lh[nz - 1] > 0.0f;
     if (metres)
         for (uint32_t k = 1; k < nz; ++k) A_CHECK(e, lh[k] >= lh[k - 1], "level heights must be non-decreasing");
-    fp::level_table(lh, nz, e->lh_host);
-    e->has_heights = metres;
+    e->has_heights = fp::level_table(lh, nz, e->lh_host);
     e->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
     e->level_seconds = seconds_between_levels;
     const uint64_t np = e->npts;

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -138,9 +139,8 @@ int validate_levels(fp_ctx *ctx, const float *lh)
     if (int rc = check_levels(ctx, lh)) return rc;
     const uint32_t nz = ctx->desc.met_nz;
     const bool metres = lh[nz - 1] > 0.0f;
-    ctx->has_heights = metres;
+    ctx->has_heights = fp::level_table(lh, nz, ctx->level_heights_host);
     ctx->zmax = metres ? lh[nz - 1] : (float)(nz - 1);
-    fp::level_table(lh, nz, ctx->level_heights_host);
     return FP_OK;
 }
 
@@ -858,7 +858,7 @@ static int build_step_args(fp_ctx *ctx, const fp_step_params *p, fp::StepArgs &a
     a.ny = ctx->desc.met_ny;
     a.nz = ctx->desc.met_nz;
     a.level_heights = ctx->level_heights;
-    a.has_heights = ctx->has_heights ? 1u : 0u;
+    a.has_heights = ctx->has_heights;
     a.zmax = ctx->zmax;
     a.pbl = ctx->pbl;
     a.precip = ctx->precip;
@@ -1054,13 +1054,18 @@ int fp_interpolation_alpha(int64_t t0_s, int64_t t1_s, int64_t target_s, int str
 int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n)
 {
     FP_CHECK(ctx, ctx && x && y && (out || n == 0), "null argument");
-    FP_CHECK(ctx, fn >= FP_PROBE_EXP && fn <= FP_PROBE_LOG_UNIT, "unknown probe function");
+    FP_CHECK(ctx, fn >= FP_PROBE_EXP && fn <= FP_PROBE_LEVEL, "unknown probe function");
+    if (fn == FP_PROBE_LEVEL && !ctx->met_ready) return fail(ctx, FP_E_STATE, "no met bracket uploaded");
     if (n == 0) return FP_OK;
     float *d = nullptr;
     FP_HIP(ctx, dalloc(&d, n * 3));
     hipError_t e = hipMemcpyAsync(d, x, n * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d + n, y, n * 4, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = fp::launch_math_probe(fn, d, d + n, d + 2 * n, n, ctx->stream);
+    if (e == hipSuccess)
+        e = fn == FP_PROBE_LEVEL
+                ? fp::launch_level_probe(ctx->level_heights, ctx->desc.met_nz, ctx->has_heights, d, d + 2 * n, n,
+                                         ctx->stream)
+                : fp::launch_math_probe(fn, d, d + n, d + 2 * n, n, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
@@ -1149,14 +1154,16 @@ int fp_set_launch(fp_ctx *ctx, uint32_t block_size, uint32_t particles_per_threa
 namespace fp {
 void set_context_free_error(const char *msg) { set_create_error(msg); }
 
-void level_table(const float *lh, uint32_t nz, std::vector<float> &out)
+uint32_t level_table(const float *lh, uint32_t nz, std::vector<float> &out)
 {
     out.assign(lh, lh + nz);
     out.resize(level_table_words(nz), 0.0f);
+    out[nz] = out[nz + 1] = std::numeric_limits<float>::infinity();
     const float top = lh[nz - 1];
     const float inv_w = top > 0.0f ? (float)kLevelBuckets / top : 0.0f;
-    out[nz] = inv_w;
+    out[nz + 2] = inv_w;
     uint16_t first[kLevelBuckets];
+    uint32_t last_first = 0;
     for (uint32_t b = 0; b < kLevelBuckets; ++b) {
         // the device maps z to band b = (uint)(z * inv_w) with one rounding of
         // the product, so every z in band b satisfies z >= b / inv_w (1 - 2^-24)
@@ -1164,7 +1171,13 @@ void level_table(const float *lh, uint32_t nz, std::vector<float> &out)
         uint32_t k = 1;
         while (k < nz && (double)lh[k] < zlow) ++k;
         first[b] = (uint16_t)std::min(k, nz);
+        last_first = std::max(last_first, (uint32_t)first[b]);
     }
-    std::memcpy(out.data() + nz + 1, first, sizeof first);
+    std::memcpy(out.data() + nz + 3, first, sizeof first);
+    if (!(top > 0.0f)) return kLevelIndex;
+    // the banded lookup (level_of_banded) needs the below-first-level quotient
+    // z / h0 (a positive h0) and every band's first candidate below the top
+    // level, so that it reads at most the two +inf words past the heights
+    return nz >= 2 && lh[0] > 0.0f && last_first <= nz - 1 ? kLevelHeightsBanded : kLevelHeights;
 }
 }  // namespace fp

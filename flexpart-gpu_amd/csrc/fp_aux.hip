@@ -248,6 +248,16 @@ __global__ void math_probe_kernel(int fn, const float *x, const float *y, float 
     out[i] = r;
 }
 
+__global__ void __launch_bounds__(256) level_probe_kernel(const float *table, uint32_t nz, uint32_t mode,
+                                                         const float *z, float *out, uint64_t n)
+{
+    __shared__ float lh[kLevelTableMax];
+    for (uint32_t k = threadIdx.x; k < level_table_words(nz); k += 256) lh[k] = table[k];
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = level_of(z[i], lh, nz, mode);
+}
+
 // Exhaustive check of the step's square root (sqrt_in_range) over the float
 // bit patterns [first, last]: the result s is the correctly rounded root of x
 // exactly when x lies strictly between the squares of the midpoints from s to
@@ -378,6 +388,14 @@ hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out,
 {
     if (n == 0) return hipSuccess;
     math_probe_kernel<<<blocks_for(n, 256), 256, 0, st>>>(fn, x, y, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_probe(const float *table, uint32_t nz, uint32_t mode, const float *z, float *out, uint64_t n,
+                              hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    level_probe_kernel<<<blocks_for(n, 256), 256, 0, st>>>(table, nz, mode, z, out, n);
     return hipGetLastError();
 }
 

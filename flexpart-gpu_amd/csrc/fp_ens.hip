@@ -28,7 +28,7 @@ struct fp_ens {
     float *met_stage = nullptr;  // 6 * npts floats for host uploads
     float *level_heights = nullptr;
     std::vector<float> lh_host;
-    bool has_heights = false;
+    uint32_t has_heights = fp::kLevelIndex;
     float zmax = 0.0f;
     std::vector<char> met_ready;
 
@@ -180,8 +180,7 @@ int met_member_common(fp_ens *e, uint32_t m, const float *const f[6], hipMemcpyK
     bool any = false;
     for (char r : e->met_ready) any = any || r;
     if (any) E_CHECK(e, std::equal(lh, lh + e->nz, e->lh_host.begin()), "members must share the level heights");
-    fp::level_table(lh, e->nz, e->lh_host);
-    e->has_heights = metres;
+    e->has_heights = fp::level_table(lh, e->nz, e->lh_host);
     e->zmax = metres ? lh[e->nz - 1] : (float)(e->nz - 1);
     E_HIP(e, hipMemcpyAsync(e->level_heights, e->lh_host.data(), e->lh_host.size() * 4, hipMemcpyHostToDevice,
                             e->stream));
@@ -470,7 +469,7 @@ int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, c
         a.met = e->met + (uint64_t)m * np * 2;
         a.nx = e->nx; a.ny = e->ny; a.nz = e->nz;
         a.level_heights = e->level_heights;
-        a.has_heights = e->has_heights ? 1u : 0u;
+        a.has_heights = e->has_heights;
         a.zmax = e->zmax;
         a.pbl = e->pbl;
         a.precip = e->precip;

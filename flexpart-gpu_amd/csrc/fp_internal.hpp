@@ -30,13 +30,19 @@ constexpr uint32_t kMaxLevels = 256;   // met levels held in LDS by the step ker
 // Particle slots per SoA set: the step kernels address a field word by a
 // 32-bit byte offset from the field's base (slot << 2).
 constexpr uint64_t kMaxSlots = 1ull << 30;
-// Level table (device memory and LDS): the nz level heights, then 1/w as a
-// float, w = top height / kLevelBuckets, then for each height band
-// [b w, (b+1) w) the first level k >= 1 whose height may be at or above a
-// height in the band (uint16, two per word); level_of starts its search there.
+// Level table (device memory and LDS): the nz level heights, two +inf words
+// (heights above the top, so that a lookup may read two levels past any
+// level), then 1/w as a float, w = top height / kLevelBuckets, then for each
+// height band [b w, (b+1) w) the first level k >= 1 whose height may be at or
+// above a height in the band (uint16, two per word); level_of starts its
+// search there.
 constexpr uint32_t kLevelBuckets = 512;
-constexpr uint32_t kLevelTableMax = kMaxLevels + 1 + kLevelBuckets / 2;
-__host__ __device__ inline uint32_t level_table_words(uint32_t nz) { return nz + 1 + kLevelBuckets / 2; }
+constexpr uint32_t kLevelTableMax = kMaxLevels + 3 + kLevelBuckets / 2;
+__host__ __device__ inline uint32_t level_table_words(uint32_t nz) { return nz + 3 + kLevelBuckets / 2; }
+// StepArgs::has_heights: levels given as indices, as heights in metres, or
+// as heights whose table admits the two-comparison lookup (a positive first
+// height, at least two levels; level_table decides).
+constexpr uint32_t kLevelIndex = 0, kLevelHeights = 1, kLevelHeightsBanded = 2;
 constexpr uint32_t kWave = 64;         // CDNA wavefront width
 constexpr uint32_t kXcds = 8;          // MI355X accelerator complex dies
 
@@ -278,7 +284,7 @@ struct fp_ctx {
     float *level_heights = nullptr;
     std::vector<float> level_heights_host;
     bool met_ready = false;
-    bool has_heights = false;
+    uint32_t has_heights = fp::kLevelIndex;  // StepArgs::has_heights
     float zmax = 0.0f;
     float *met_stage = nullptr;    // 6 * npoints floats for host uploads
 
@@ -389,6 +395,9 @@ hipError_t launch_philox_blocks(const uint32_t counter[4], const uint32_t key[2]
                                 uint32_t *out, hipStream_t st);
 hipError_t launch_math_probe(int fn, const float *x, const float *y, float *out, uint64_t n,
                              hipStream_t st);
+// level_of of n heights z over a level table (level_table_words(nz) words).
+hipError_t launch_level_probe(const float *table, uint32_t nz, uint32_t mode, const float *z, float *out, uint64_t n,
+                              hipStream_t st);
 hipError_t launch_math_sweep(int fn, uint32_t first, uint32_t last, unsigned long long *fails,
                              uint32_t *first_fail, hipStream_t st);
 // Source-term injection (fp_release.hip).
@@ -425,8 +434,9 @@ hipError_t sort_morton_range(SortWork &w, const fp_sort_params &p, const Soa &in
 void set_context_free_error(const char *msg);
 
 // The level table of nz non-decreasing heights (level-index mode when the top
-// height is not positive), level_table_words(nz) words (fp_api.hip).
-void level_table(const float *lh, uint32_t nz, std::vector<float> &out);
+// height is not positive), level_table_words(nz) words; returns the
+// StepArgs::has_heights mode the table is read in (fp_api.hip).
+uint32_t level_table(const float *lh, uint32_t nz, std::vector<float> &out);
 
 // Shared by the single-run and ensemble entry points (fp_api.hip).
 // Scalar step arguments (timestep, scales, Philox key and counter, sub-steps,

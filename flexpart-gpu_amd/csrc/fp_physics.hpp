@@ -148,13 +148,12 @@ __device__ __forceinline__ Axis axis_of(float coord, uint32_t n)
 // LDS.  The search starts at the first level that can qualify for z's height
 // band (the table's band index) and steps up from there: every level below
 // the band's start is lower than z, so the result is that of the full scan.
-__device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz, bool heights)
+__device__ __forceinline__ float level_of_scan(float z, const float *lh, uint32_t nz)
 {
-    if (!heights) return z;
     const float h0 = lh[0];
     if (z <= h0) return h0 > 0.0f ? z / h0 : 0.0f;
-    const float inv_w = lh[nz];
-    const uint16_t *first = reinterpret_cast<const uint16_t *>(lh + nz + 1);
+    const float inv_w = lh[nz + 2];
+    const uint16_t *first = reinterpret_cast<const uint16_t *>(lh + nz + 3);
     const uint32_t b = (uint32_t)clampq(z * inv_w, 0.0f, (float)(kLevelBuckets - 1u));
     uint32_t lo = first[b];
     while (lo < nz && !(z <= lh[lo])) ++lo;
@@ -162,6 +161,47 @@ __device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz,
     const float hp = lh[lo - 1];
     const float den = lh[lo] - hp;
     return den > 0.0f ? (float)(lo - 1) + (z - hp) / den : (float)(lo - 1);
+}
+
+// The same lookup for a table with h0 > 0, at least two levels and every
+// band's first candidate f below the top level (kLevelHeightsBanded): the
+// heights are non-decreasing, so the levels below z among f and f + 1 are a
+// prefix and two comparisons give the scan's answer whenever it lies within
+// two levels of f (the words past the top level are +inf).  A lane whose
+// answer lies further up (a band that holds more levels, or z NaN) continues
+// the scan from there.  z <= h0 gives f = 1 and the answer 1, so one
+// division serves both cases: (z - 0) / (h0 - 0) = z / h0, and -0 + q = q.
+// With non-decreasing heights the scan's den = h[lo] - h[lo - 1] is positive
+// whenever it stops at lo < nz, so its den > 0 test cannot fail here.
+__device__ __forceinline__ float level_of_banded(float z, const float *lh, uint32_t nz)
+{
+    const float h0 = lh[0];
+    const bool below = z <= h0;
+    const float inv_w = lh[nz + 2];
+    const uint16_t *first = reinterpret_cast<const uint16_t *>(lh + nz + 3);
+    const uint32_t b = (uint32_t)clampq(z * inv_w, 0.0f, (float)(kLevelBuckets - 1u));
+    const uint32_t f = first[b];
+    const float c0 = lh[f], c1 = lh[f + 1u];
+    uint32_t lo = f + (!(z <= c0) ? 1u : 0u) + (!(z <= c1) ? 1u : 0u);
+    float hp = lh[lo - 1u], hh = lh[lo];
+    if (!(z <= hh)) {
+        while (lo < nz && !(z <= lh[lo])) ++lo;
+        hp = lh[lo - 1u];
+        hh = lh[lo];
+    }
+    const float lp = below ? 0.0f : hp;
+    const float lt = below ? hp : hh;
+    const float q = (z - lp) / (lt - lp);
+    const float base = below ? -0.0f : (float)(lo - 1u);
+    return lo < nz ? base + q : (float)(nz - 1u);
+}
+
+// heights: kLevelIndex (z is a level index already), kLevelHeights or
+// kLevelHeightsBanded (the table's lookup, identical results)
+__device__ __forceinline__ float level_of(float z, const float *lh, uint32_t nz, uint32_t heights)
+{
+    if (heights == kLevelIndex) return z;
+    return heights == kLevelHeightsBanded ? level_of_banded(z, lh, nz) : level_of_scan(z, lh, nz);
 }
 
 struct Wind { float u, v, w; };
@@ -176,7 +216,7 @@ __device__ __forceinline__ Wind sample_wind(const StepArgs &a, const float *lh, 
     const Axis ay = axis_of(y, a.ny, a.nym);
     // with heights, level_of returns the result of arithmetic; in level-index
     // mode it returns z as it is
-    const float lz = level_of(z, lh, a.nz, a.has_heights != 0);
+    const float lz = level_of(z, lh, a.nz, a.has_heights);
     const Axis az = a.has_heights ? axis_of(lz, a.nz, a.nzm) : axis_of<true>(lz, a.nz, a.nzm);
     const uint32_t ny = a.ny, nz = a.nz;
     const uint32_t r00 = (ax.lo * ny + ay.lo) * nz, r10 = (ax.hi * ny + ay.lo) * nz;

@@ -433,7 +433,10 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
 /* Evaluates one of the step kernel's arithmetic primitives on the device for
  * n inputs (host arrays; y is used by pow and div).  Diagnostic: lets a host
  * check that the device rounds exactly as the CPU does.  No counterpart in
- * the reference.  fn: FP_PROBE_*.  Blocking. */
+ * the reference.  fn: FP_PROBE_*; FP_PROBE_LEVEL is the step's height ->
+ * fractional level lookup (meters_to_grid_level) of the heights x over the
+ * level table of the context's met bracket (FP_E_STATE before the first met
+ * upload).  Blocking. */
 #define FP_PROBE_EXP 0
 #define FP_PROBE_LOG 1
 #define FP_PROBE_SIN 2
@@ -442,6 +445,7 @@ int fp_philox_blocks(fp_ctx *ctx, const uint32_t counter[4], const uint32_t key[
 #define FP_PROBE_SQRT 5
 #define FP_PROBE_DIV 6
 #define FP_PROBE_LOG_UNIT 7
+#define FP_PROBE_LEVEL 8
 int fp_math_probe(fp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n);
 
 /* Exhaustive device check of one of the step's arithmetic primitives over the

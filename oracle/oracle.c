@@ -852,6 +852,20 @@ void orc_math_eval(int fn, const float *x, const float *y, float *out, uint64_t 
     }
 }
 
+/* meters_to_grid_level over a level table, for the lookup's parity tests */
+void orc_level_eval(const float *level_heights, uint32_t nz, const float *z, float *out, uint64_t n)
+{
+    orc_met m;
+    memset(&m, 0, sizeof m);
+    m.nz = nz;
+    m.level_heights = level_heights;
+    adv_ctx a;
+    a.m = &m;
+    a.alpha = 0.0f;
+    a.has_heights = level_heights[nz - 1] > 0.0f;
+    for (uint64_t i = 0; i < n; ++i) out[i] = meters_to_grid_level(&a, z[i]);
+}
+
 /* ---------------------------------------------------------------- release */
 
 /* target_emitted_count (release/mod.rs:181-197) */

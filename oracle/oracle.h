@@ -107,6 +107,9 @@ int orc_sort_morton(const orc_particle *p, uint64_t n, uint32_t bits, int explic
                     const double bounds[6], orc_particle *out, uint32_t *new_to_old);
 
 void orc_math_eval(int fn, const float *x, const float *y, float *out, uint64_t n);
+/* meters_to_grid_level (advection_dual_wind.wgsl:94-117) of n heights over
+ * the nz-level table (level-index mode when its top is not positive). */
+void orc_level_eval(const float *level_heights, uint32_t nz, const float *z, float *out, uint64_t n);
 
 /* ---- source-term injection (release/mod.rs) --------------------------- */
 

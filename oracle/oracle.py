@@ -111,6 +111,7 @@ def lib(variant: str = "fpm") -> C.CDLL:
                                       C.POINTER(C.c_double), C.c_void_p, _u32]
         L.orc_sort_morton.restype = C.c_int
         L.orc_math_eval.argtypes = [C.c_int, _f, _f, _f, C.c_uint64]
+        L.orc_level_eval.argtypes = [_f, C.c_uint32, _f, _f, C.c_uint64]
         _LIBS[variant] = L
     return _LIBS[variant]
 
@@ -370,6 +371,16 @@ def math_eval(fn: str, x, y=None):
     y = _arr(np.zeros_like(x) if y is None else y, np.float32)
     out = np.zeros_like(x)
     lib().orc_math_eval(MATH_FN[fn], _fp(x), _fp(y), _fp(out), len(x))
+    return out
+
+
+def level_eval(level_heights, z):
+    """meters_to_grid_level of each height z over the level table
+    (advection_dual_wind.wgsl:94-117)."""
+    lh = _arr(level_heights, np.float32)
+    z = _arr(z, np.float32)
+    out = np.zeros_like(z)
+    lib().orc_level_eval(_fp(lh), len(lh), _fp(z), _fp(out), len(z))
     return out
 
 

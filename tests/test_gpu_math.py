@@ -108,3 +108,52 @@ def test_step_sqrt_sweep_detects_misrounding(ctx):
     must report failures there (the exact test has teeth)."""
     fails, first = ctx.math_sweep("sqrt_step", 0x00800000, 0x01000000)
     assert fails > 0 and 0x00800000 <= first <= 0x01000000
+
+
+def _level_tables():
+    k = np.arange(60, dtype=np.float64)
+    workload = (15000.0 * (k / 59) ** 2 + 10.0).astype(np.float32)
+    # levels crowded near the ground: the first bands hold up to 40 levels
+    crowded = np.concatenate([0.25 + 0.05 * np.arange(40), 40.0 + 250.0 * np.arange(1, 21)]).astype(np.float32)
+    # repeated heights (the scan stops at the first of a run)
+    ties = np.array([10, 10, 20, 20, 20, 50, 50, 400, 400, 401, 5000, 5000], np.float32)
+    ground = workload.copy()
+    ground[0] = 0.0       # a first level at 0 m: the scan lookup
+    return {"workload": workload, "crowded": crowded, "ties": ties, "ground": ground,
+            "two": np.array([5.0, 3000.0], np.float32), "index": np.zeros(8, np.float32)}
+
+
+def _level_queries(lh, rng):
+    top = float(max(lh[-1], 1.0))
+    edges = np.asarray(lh, np.float32)
+    near = np.concatenate([edges, np.nextafter(edges, np.float32(np.inf)), np.nextafter(edges, np.float32(-np.inf))])
+    inv_w = np.float32(512.0) / np.float32(top)
+    bands = (np.arange(513, dtype=np.float32) / inv_w).astype(np.float32)
+    near_bands = np.concatenate([bands, np.nextafter(bands, np.float32(np.inf)),
+                                 np.nextafter(bands, np.float32(-np.inf))])
+    special = np.array([0.0, -0.0, 1e-30, -1e-30, 1e-45, -5.0, np.nan, np.inf, -np.inf, 3e38, -3e38,
+                        top, top * 1.5], np.float32)
+    return np.concatenate([rng.uniform(-0.1 * top, 1.2 * top, 200_000).astype(np.float32),
+                           rng.uniform(0.0, float(edges[min(3, len(edges) - 1)]) + 1.0, 50_000).astype(np.float32),
+                           near, near_bands, special]).astype(np.float32)
+
+
+@pytest.mark.parametrize("table", ["workload", "crowded", "ties", "ground", "two", "index"])
+def test_level_lookup_matches_oracle(fp, orc, table):
+    """The step's height -> fractional level lookup equals the reference's
+    linear scan (meters_to_grid_level) bit for bit: at, just above and just
+    below every level and every band edge of the lookup's band index, below
+    the first level, above the top, for NaN, infinities and signed zeros, on
+    tables that take the two-comparison path (workload heights, crowded low
+    levels that overflow it into the scan, repeated heights, two levels), the
+    plain scan (first level at 0 m) and level-index mode."""
+    lh = _level_tables()[table]
+    nz = lh.size
+    z = _level_queries(lh, np.random.default_rng(21))
+    with fp.GpuTimestep(16, (4, 4, nz), n_species=1) as c:
+        met = np.zeros(16 * nz, np.float32)
+        c.upload_met_bracket(met, met, met, met, met, met, lh)
+        gpu = c.math_probe("level", z)
+    cpu = orc.level_eval(lh, z)
+    ok, bad = bits_equal(gpu, cpu)
+    assert ok, f"{table}: {bad} of {z.size} differ"

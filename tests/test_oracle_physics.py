@@ -417,3 +417,25 @@ def test_deterministic_math_oracle_tracks_the_libm_oracle(orc, small):
     assert np.quantile(dx, 0.999) < 1e-4
     assert np.quantile(dz, 0.99) < 1e-2
     assert np.mean(dz < 1e-2) > 0.99
+
+
+def test_level_lookup_known_answers(orc):
+    """meters_to_grid_level (advection_dual_wind.wgsl:94-117) at hand-worked
+    points: z / h0 below the first level, k - 1 + (z - h[k-1]) / (h[k] -
+    h[k-1]) at the first k >= 1 with z <= h[k], the top index above the
+    table, the first of a run of equal heights, and z itself in level-index
+    mode (top height not positive)."""
+    lh = np.array([10.0, 20.0, 40.0, 80.0], np.float32)
+    z = np.array([5.0, 10.0, 15.0, 20.0, 30.0, 70.0, 80.0, 90.0, -4.0, np.nan], np.float32)
+    want = np.array([0.5, 1.0, 0.5, 1.0, 1.5, 2.75, 3.0, 3.0, -0.4, 3.0], np.float32)
+    assert np.array_equal(orc.level_eval(lh, z), want)
+    ties = np.array([10.0, 10.0, 20.0, 20.0, 50.0], np.float32)
+    z = np.array([10.0, 15.0, 20.0, 35.0], np.float32)
+    assert np.array_equal(orc.level_eval(ties, z), np.array([1.0, 1.5, 2.0, 3.5], np.float32))
+    # a first level at 0 m: heights at or below it map to level 0
+    ground = np.array([0.0, 100.0], np.float32)
+    assert np.array_equal(orc.level_eval(ground, np.array([-3.0, 0.0, 50.0], np.float32)),
+                          np.array([0.0, 0.0, 0.5], np.float32))
+    index = np.zeros(4, np.float32)
+    assert np.array_equal(orc.level_eval(index, np.array([1.25, -2.0], np.float32)),
+                          np.array([1.25, -2.0], np.float32))
