@@ -435,6 +435,11 @@ struct TurbRegime {
     // each quotient of the profiles and of the Langevin updates needs no
     // range scaling (lean_lane)
     bool lean;
+    // wavefront-uniform: with lean, the regime every active lane of the Hanna
+    // evaluation shares (0 neutral, 1 unstable, 2 stable), else -1; it holds
+    // for the later profile evaluations of the step, whose active lanes are
+    // the same
+    int wr;
 };
 template <class T> struct TurbT : TurbRegime {
     T sigu, sigv, sigw, dsigwdz, tlu, tlv, tlw;
@@ -537,7 +542,8 @@ __device__ __forceinline__ void vertical_profile_r(const TurbRegime &t, T z_in, 
 }
 
 // The regime-selected evaluation: a wavefront whose active lanes share one
-// regime and pass the range check (t.lean) runs that regime's instantiation
+// regime and pass the range check (t.wr >= 0, decided once per step by
+// hanna_full) runs that regime's instantiation
 // with unscaled quotients (no per-lane selects between the regimes'
 // operands); any other wavefront runs the per-lane selected one with IEEE
 // quotients.  The arithmetic of each regime is the same in every
@@ -546,12 +552,9 @@ template <class T>
 __device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, double log_1em3, T &sigw,
                                                  T &dsigwdz, T &tlw)
 {
-    if (t.lean) {
-        const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
-        if (n == live) return vertical_profile_r<0, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
-        if (u == live) return vertical_profile_r<1, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
-        if ((n | u) == 0ull) return vertical_profile_r<2, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
-    }
+    if (t.wr == 0) return vertical_profile_r<0, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+    if (t.wr == 1) return vertical_profile_r<1, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+    if (t.wr == 2) return vertical_profile_r<2, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
     vertical_profile_r<-1, false>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
 }
 
@@ -654,12 +657,14 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
     t.neu = regime == 0u;
     t.uns = regime == 1u;
     t.lean = launch_lean && __ballot(!lean_lane(t.ust, t.wst, t.h, val(z_in), t.uns, t.ol)) == 0ull;
+    t.wr = -1;
     if (t.lean) {
         const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
-        if (n == live) return hanna_full_r<0, true>(t, z_in, log_1em3);
-        if (u == live) return hanna_full_r<1, true>(t, z_in, log_1em3);
-        if ((n | u) == 0ull) return hanna_full_r<2, true>(t, z_in, log_1em3);
+        t.wr = n == live ? 0 : (u == live ? 1 : ((n | u) == 0ull ? 2 : -1));
     }
+    if (t.wr == 0) return hanna_full_r<0, true>(t, z_in, log_1em3);
+    if (t.wr == 1) return hanna_full_r<1, true>(t, z_in, log_1em3);
+    if (t.wr == 2) return hanna_full_r<2, true>(t, z_in, log_1em3);
     return hanna_full_r<-1, false>(t, z_in, log_1em3);
 }
 
@@ -744,14 +749,25 @@ __device__ __forceinline__ T ou_vertical(T prev, T sw, T dsigwdz, T tl, float dt
 // PBL reflection (langevin_fused.wgsl:481-500)
 __device__ __forceinline__ void reflect(float &pz, float &tw, int32_t &cbt, float zmin, float h)
 {
-    if (!(h > zmin)) return;
+    // Branch-free form of the reference loop: every lane evaluates each bounce
+    // with selects and the wavefront leaves when no lane has a bounce left.  A
+    // lane inside [z_min, h] (or NaN) is left as it is, which is where the
+    // reference's loop breaks; a lane with h <= z_min is not touched, its early
+    // return.
+    const bool on = h > zmin;
+    const float lo2 = 2.0f * zmin, hi2 = 2.0f * h;
     float z = pz;
     for (int bounce = 0; bounce < 10; ++bounce) {
-        if (z < zmin) { z = 2.0f * zmin - z; cbt = -cbt; tw = -tw; }
-        else if (z > h) { z = 2.0f * h - z; cbt = -cbt; tw = -tw; }
-        else break;
+        // with h > z_min, z > h excludes z < z_min (the reference's else-if)
+        const bool below = on && z < zmin;
+        const bool above = on && z > h;
+        if (__ballot(below || above) == 0ull) break;
+        z = below ? lo2 - z : (above ? hi2 - z : z);
+        const bool flip = below || above;
+        cbt = flip ? -cbt : cbt;
+        tw = flip ? -tw : tw;
     }
-    pz = clampq(z, zmin, h);
+    if (on) pz = clampq(z, zmin, h);
 }
 
 // ---------------------------------------------------------------- gridding
