@@ -139,7 +139,7 @@ int capture_steps(fp_ens *e, int s, uint32_t n_steps)
     hipError_t h = hipSuccess;
     for (uint32_t k = 0; k < n_steps && h == hipSuccess; ++k)
         h = fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt, e->sub4,
-                                e->stream);
+                                e->soa.capacity <= fp::kSoa32Slots, e->stream);
     hipGraph_t g = nullptr;
     const hipError_t h2 = hipStreamEndCapture(e->stream, &g);
     if (h != hipSuccess || h2 != hipSuccess) {
@@ -526,7 +526,7 @@ int fp_ens_step(fp_ens *e, uint32_t n_steps, int use_graph, uint32_t *counters_o
     } else {
         for (uint32_t k = 0; k < n_steps; ++k)
             E_HIP(e, fp::launch_ens_step(e->d_members, e->d_blocks, e->d_clock, e->nblocks, e->block, e->ppt,
-                                         e->sub4, e->stream));
+                                         e->sub4, e->soa.capacity <= fp::kSoa32Slots, e->stream));
     }
     E_HIP(e, hipEventRecord(e->ev_b, e->stream));
     E_HIP(e, hipEventSynchronize(e->ev_b));

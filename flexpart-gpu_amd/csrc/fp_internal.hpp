@@ -89,6 +89,23 @@ struct Soa {
         using G = __attribute__((address_space(1))) char *;
         return *reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + off);
     }
+    // The same word in a set of at most kSoa32Slots slots (at most 2^32
+    // bytes): addressed from the set's base with the 32-bit byte offset
+    // field * capacity * 4 + slot * 4, one scalar multiply and one vector add
+    // per access instead of a 64-bit field address computed per access.
+    template <class T> __device__ __attribute__((address_space(1))) T &w32(uint32_t field, uint32_t slot) const
+    {
+        const uint64_t p = reinterpret_cast<uint64_t>(base);
+        const uint32_t stride = __builtin_amdgcn_readfirstlane((uint32_t)capacity << 2);
+        // the slot's byte offset is opaque, so that the compiler adds the
+        // field's offset to it (one add) rather than rescaling a slot sum
+        uint32_t s4 = slot << 2;
+        asm("" : "+v"(s4));
+        uint32_t off = s4 + field * stride;
+        asm("" : "+v"(off));
+        using G = __attribute__((address_space(1))) char *;
+        return *reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<G>(p) + off);
+    }
 #else
     template <class T> __host__ __device__ T &w(uint32_t field, uint32_t slot) const
     {
@@ -96,6 +113,9 @@ struct Soa {
     }
 #endif
 };
+// Capacity up to which a set's words are addressed with 32-bit byte offsets
+// from its base (Soa::w32): F_COUNT * capacity * 4 <= 2^32.
+constexpr uint64_t kSoa32Slots = (1ull << 32) / (F_COUNT * 4ull);
 
 // Largest met grid (points) of a context, ensemble member or adjoint run:
 // the packed two-time bracket (32 B per point) stays below 4 GiB, so a gather
@@ -369,9 +389,10 @@ hipError_t launch_step(const StepArgs &a, uint32_t block, hipStream_t st);
 hipError_t launch_step_gathered(const StepArgs &a, bool at_source_slot, hipStream_t st);
 // one step of slots [0, a.n) of a.src into the same slots of a.s (stepped words)
 hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st);
-// ppt: particles per thread of every member (a.ppt), selecting the kernel variant
+// ppt: particles per thread of every member (a.ppt), selecting the kernel
+// variant; soa32: the ensemble's particle set holds at most kSoa32Slots slots
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, hipStream_t st);
+                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, bool soa32, hipStream_t st);
 hipError_t launch_pack_met(const float *u0, const float *v0, const float *w0, const float *u1,
                            const float *v1, const float *w1, float4 *met, uint32_t npts,
                            hipStream_t st);

@@ -31,6 +31,23 @@ __device__ __forceinline__ bool stepped_field(uint32_t f, uint32_t n_species)
            (f >= F_MASS0 && f < F_MASS0 + n_species);
 }
 
+// A word of a particle set: with SOA32 (a set of at most kSoa32Slots slots)
+// addressed with a 32-bit offset from the set's base, else from the field's
+// own base.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <bool SOA32, class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T &soa_word(const Soa &s, uint32_t field, uint32_t slot)
+{
+    if constexpr (SOA32) return s.w32<T>(field, slot);
+    else return s.w<T>(field, slot);
+}
+#else
+template <bool SOA32, class T> __host__ __device__ T &soa_word(const Soa &s, uint32_t field, uint32_t slot)
+{
+    return s.w<T>(field, slot);
+}
+#endif
+
 // One block's chunk of the step: slots first + r*BLOCK + threadIdx.x for
 // r < a.ppt, temporal weight alpha and Philox base counter ctr.  With GATHER,
 // slot i of a.s receives the step of slot a.gather_map[i] of a.src, taken at
@@ -38,7 +55,7 @@ __device__ __forceinline__ bool stepped_field(uint32_t f, uint32_t n_species)
 // step) or at its source slot (kGatherFromSlot: the record is moved there by a
 // compaction after the step) - the slot that selects its Philox counters and
 // per-slot forcing.
-template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0, bool SOA32 = false>
 __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, float alpha, U4 ctr,
                                            const float *lh, const float *oh, GridLds &gt)
 {
@@ -64,14 +81,14 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
         float px = 0.0f, py = 0.0f, pz = 0.0f, tu = 0.0f, tv = 0.0f, tw = 0.0f;
         float mass[FP_MAX_SPECIES];
         if (inb) {
-            flags = sl.w<uint32_t>(F_FLAGS, (uint32_t)g);
-            cx = sl.w<int32_t>(F_CELL_X, (uint32_t)g); cy = sl.w<int32_t>(F_CELL_Y, (uint32_t)g);
-            px = sl.w<float>(F_POS_X, (uint32_t)g); py = sl.w<float>(F_POS_Y, (uint32_t)g); pz = sl.w<float>(F_POS_Z, (uint32_t)g);
-            tu = sl.w<float>(F_TURB_U, (uint32_t)g); tv = sl.w<float>(F_TURB_V, (uint32_t)g); tw = sl.w<float>(F_TURB_W, (uint32_t)g);
-            cbt = sl.w<int32_t>(F_CBT, (uint32_t)g);
+            flags = soa_word<SOA32, uint32_t>(sl, F_FLAGS, (uint32_t)g);
+            cx = soa_word<SOA32, int32_t>(sl, F_CELL_X, (uint32_t)g); cy = soa_word<SOA32, int32_t>(sl, F_CELL_Y, (uint32_t)g);
+            px = soa_word<SOA32, float>(sl, F_POS_X, (uint32_t)g); py = soa_word<SOA32, float>(sl, F_POS_Y, (uint32_t)g); pz = soa_word<SOA32, float>(sl, F_POS_Z, (uint32_t)g);
+            tu = soa_word<SOA32, float>(sl, F_TURB_U, (uint32_t)g); tv = soa_word<SOA32, float>(sl, F_TURB_V, (uint32_t)g); tw = soa_word<SOA32, float>(sl, F_TURB_W, (uint32_t)g);
+            cbt = soa_word<SOA32, int32_t>(sl, F_CBT, (uint32_t)g);
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) mass[q] = sl.w<float>(F_MASS0 + q, (uint32_t)g);
+                if (q < a.n_species) mass[q] = soa_word<SOA32, float>(sl, F_MASS0 + q, (uint32_t)g);
         }
 
         const bool live = inb && (flags & FLAG_ACTIVE);
@@ -175,15 +192,15 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             if (a.loss_mode == FP_LOSS_MIN_MASS && mass[0] < a.loss_min_mass) flags &= ~FLAG_ACTIVE;
 
             // ---- store
-            s.w<int32_t>(F_CELL_X, (uint32_t)i) = cx; s.w<int32_t>(F_CELL_Y, (uint32_t)i) = cy;
-            s.w<float>(F_POS_X, (uint32_t)i) = px; s.w<float>(F_POS_Y, (uint32_t)i) = py; s.w<float>(F_POS_Z, (uint32_t)i) = pz;
-            s.w<float>(F_TURB_U, (uint32_t)i) = tu; s.w<float>(F_TURB_V, (uint32_t)i) = tv; s.w<float>(F_TURB_W, (uint32_t)i) = tw;
-            s.w<float>(F_VEL_U, (uint32_t)i) = cu; s.w<float>(F_VEL_V, (uint32_t)i) = cv; s.w<float>(F_VEL_W, (uint32_t)i) = cw;
-            s.w<int32_t>(F_CBT, (uint32_t)i) = cbt;
-            s.w<uint32_t>(F_FLAGS, (uint32_t)i) = flags;
+            soa_word<SOA32, int32_t>(s, F_CELL_X, (uint32_t)i) = cx; soa_word<SOA32, int32_t>(s, F_CELL_Y, (uint32_t)i) = cy;
+            soa_word<SOA32, float>(s, F_POS_X, (uint32_t)i) = px; soa_word<SOA32, float>(s, F_POS_Y, (uint32_t)i) = py; soa_word<SOA32, float>(s, F_POS_Z, (uint32_t)i) = pz;
+            soa_word<SOA32, float>(s, F_TURB_U, (uint32_t)i) = tu; soa_word<SOA32, float>(s, F_TURB_V, (uint32_t)i) = tv; soa_word<SOA32, float>(s, F_TURB_W, (uint32_t)i) = tw;
+            soa_word<SOA32, float>(s, F_VEL_U, (uint32_t)i) = cu; soa_word<SOA32, float>(s, F_VEL_V, (uint32_t)i) = cv; soa_word<SOA32, float>(s, F_VEL_W, (uint32_t)i) = cw;
+            soa_word<SOA32, int32_t>(s, F_CBT, (uint32_t)i) = cbt;
+            soa_word<SOA32, uint32_t>(s, F_FLAGS, (uint32_t)i) = flags;
 #pragma unroll
             for (uint32_t q = 0; q < FP_MAX_SPECIES; ++q)
-                if (q < a.n_species) s.w<float>(F_MASS0 + q, (uint32_t)i) = mass[q];
+                if (q < a.n_species) soa_word<SOA32, float>(s, F_MASS0 + q, (uint32_t)i) = mass[q];
 
             if (a.grid_fused && (flags & FLAG_ACTIVE)) {
                 grid_valid = true;
@@ -195,7 +212,7 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
             // a slot the step leaves unchanged still moves its stepped words
 #pragma unroll
             for (uint32_t f = 0; f < F_COUNT; ++f)
-                if (stepped_field(f, a.n_species)) s.w<uint32_t>(f, (uint32_t)i) = sl.w<uint32_t>(f, (uint32_t)g);
+                if (stepped_field(f, a.n_species)) soa_word<SOA32, uint32_t>(s, f, (uint32_t)i) = soa_word<SOA32, uint32_t>(sl, f, (uint32_t)g);
         }
         if (a.grid_fused) grid_reduce_wave(gt, a.grid, grid_valid, grid_c, grid_q);
     }
@@ -226,7 +243,7 @@ constexpr int step_waves_per_eu(uint32_t rows)
 {
     return rows != 1u ? 4 : (GATHER == kGatherNone && BLOCK <= 256u ? 7 : 5);
 }
-template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0>
+template <uint32_t BLOCK, int NSUB, int GATHER = kGatherNone, uint32_t ROWS = 0, bool SOA32 = false>
 __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<BLOCK, NSUB, GATHER>(ROWS)))) step_kernel(StepArgs a)
 {
@@ -242,7 +259,7 @@ __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<BLOCK, NSUB, GATHER>(ROWS))
     const uint32_t chunk = GATHER == kGatherNone && BLOCK >= 512u ? xcd_chunk_rounds(blockIdx.x, gridDim.x, 4u)
                                                                   : xcd_chunk(blockIdx.x, gridDim.x);
     const uint64_t first = (uint64_t)chunk * BLOCK * a.ppt;
-    step_chunk<BLOCK, NSUB, GATHER, ROWS>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
+    step_chunk<BLOCK, NSUB, GATHER, ROWS, SOA32>(a, first, a.alpha, U4{a.c0, a.c1, a.c2, a.c3}, lh, oh, gt);
     if (a.grid_fused) grid_lds_flush<BLOCK>(gt, a.grid);
 }
 
@@ -266,7 +283,7 @@ __global__ void ens_clock_kernel(EnsClock *c)
     c->next_step = k + 1u;
 }
 
-template <uint32_t BLOCK, int NSUB, uint32_t ROWS = 0>
+template <uint32_t BLOCK, int NSUB, uint32_t ROWS = 0, bool SOA32 = false>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS == 1 ? 5 : 4)))
 ens_step_kernel(const EnsMember *__restrict__ members, const EnsBlock *__restrict__ blocks, const EnsClock *__restrict__ clock)
 {
@@ -287,7 +304,7 @@ ens_step_kernel(const EnsMember *__restrict__ members, const EnsBlock *__restric
     load_tables<BLOCK>(members[0].a, lh, oh);  // one vertical grid for all members
     const U4 ctr = ctr_add(m.base_ctr[0], m.base_ctr[1], m.base_ctr[2], m.base_ctr[3],
                            (uint64_t)clock->step * m.ctr_per_step);
-    step_chunk<BLOCK, NSUB, kGatherNone, ROWS>(m.a, eb.first, clock->alpha, ctr, lh, oh, gt);
+    step_chunk<BLOCK, NSUB, kGatherNone, ROWS, SOA32>(m.a, eb.first, clock->alpha, ctr, lh, oh, gt);
     if (m.a.grid_fused) grid_lds_flush<BLOCK>(gt, m.a.grid);
 }
 
@@ -319,9 +336,12 @@ __global__ void __launch_bounds__(BLOCK) grid_kernel(Soa s, uint32_t n, GridArgs
 template <uint32_t BLOCK>
 hipError_t launch_block(const StepArgs &a, uint32_t grid, hipStream_t st)
 {
-    // one particle per thread: the single-row variant fits 5 wavefronts per SIMD
+    // one particle per thread: the single-row variant; the production shape
+    // (4 sub-steps) with 32-bit word offsets for sets of up to kSoa32Slots
     if (a.ppt == 1u) {
-        if (a.n_substeps == 4u) step_kernel<BLOCK, 4, kGatherNone, 1><<<grid, BLOCK, 0, st>>>(a);
+        if (a.n_substeps == 4u && a.s.capacity <= kSoa32Slots)
+            step_kernel<BLOCK, 4, kGatherNone, 1, true><<<grid, BLOCK, 0, st>>>(a);
+        else if (a.n_substeps == 4u) step_kernel<BLOCK, 4, kGatherNone, 1><<<grid, BLOCK, 0, st>>>(a);
         else step_kernel<BLOCK, -1, kGatherNone, 1><<<grid, BLOCK, 0, st>>>(a);
     } else if (a.n_substeps == 4u) {
         step_kernel<BLOCK, 4><<<grid, BLOCK, 0, st>>>(a);
@@ -362,7 +382,9 @@ hipError_t launch_gathered(const StepArgs &a, hipStream_t st)
     const uint64_t per_block = 256ull * a.ppt;
     const uint32_t grid = (uint32_t)((a.n + per_block - 1) / per_block);
     if (a.ppt == 1u) {
-        if (a.n_substeps == 4u) step_kernel<256, 4, GATHER, 1><<<grid, 256, 0, st>>>(a);
+        if (a.n_substeps == 4u && a.s.capacity <= kSoa32Slots && a.src.capacity <= kSoa32Slots)
+            step_kernel<256, 4, GATHER, 1, true><<<grid, 256, 0, st>>>(a);
+        else if (a.n_substeps == 4u) step_kernel<256, 4, GATHER, 1><<<grid, 256, 0, st>>>(a);
         else step_kernel<256, -1, GATHER, 1><<<grid, 256, 0, st>>>(a);
     } else if (a.n_substeps == 4u) {
         step_kernel<256, 4, GATHER><<<grid, 256, 0, st>>>(a);
@@ -390,15 +412,16 @@ hipError_t launch_step_out_of_place(const StepArgs &a, hipStream_t st)
 
 template <uint32_t BLOCK>
 hipError_t launch_ens_block(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                            uint32_t nblocks, uint32_t ppt, bool sub4, hipStream_t st)
+                            uint32_t nblocks, uint32_t ppt, bool sub4, bool soa32, hipStream_t st)
 {
     ens_clock_kernel<<<1, 64, 0, st>>>(clock);
     if (nblocks == 0) return hipGetLastError();
-    // one particle per thread: the single-row variant fits 5 wavefronts per
-    // SIMD (from 128 threads per block; a 64-thread block would spill)
+    // one particle per thread: the single-row variant (from 128 threads per
+    // block; a 64-thread block would spill)
     if constexpr (BLOCK >= 128) {
         if (sub4 && ppt == 1u) {
-            ens_step_kernel<BLOCK, 4, 1><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
+            if (soa32) ens_step_kernel<BLOCK, 4, 1, true><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
+            else ens_step_kernel<BLOCK, 4, 1><<<nblocks, BLOCK, 0, st>>>(members, blocks, clock);
             return hipGetLastError();
         }
     }
@@ -408,14 +431,14 @@ hipError_t launch_ens_block(const EnsMember *members, const EnsBlock *blocks, En
 }
 
 hipError_t launch_ens_step(const EnsMember *members, const EnsBlock *blocks, EnsClock *clock,
-                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, hipStream_t st)
+                           uint32_t nblocks, uint32_t block, uint32_t ppt, bool sub4, bool soa32, hipStream_t st)
 {
     switch (block) {
-    case 64: return launch_ens_block<64>(members, blocks, clock, nblocks, ppt, sub4, st);
-    case 128: return launch_ens_block<128>(members, blocks, clock, nblocks, ppt, sub4, st);
-    case 256: return launch_ens_block<256>(members, blocks, clock, nblocks, ppt, sub4, st);
-    case 512: return launch_ens_block<512>(members, blocks, clock, nblocks, ppt, sub4, st);
-    case 1024: return launch_ens_block<1024>(members, blocks, clock, nblocks, ppt, sub4, st);
+    case 64: return launch_ens_block<64>(members, blocks, clock, nblocks, ppt, sub4, soa32, st);
+    case 128: return launch_ens_block<128>(members, blocks, clock, nblocks, ppt, sub4, soa32, st);
+    case 256: return launch_ens_block<256>(members, blocks, clock, nblocks, ppt, sub4, soa32, st);
+    case 512: return launch_ens_block<512>(members, blocks, clock, nblocks, ppt, sub4, soa32, st);
+    case 1024: return launch_ens_block<1024>(members, blocks, clock, nblocks, ppt, sub4, soa32, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -161,6 +161,37 @@ def test_step_at_the_quotient_range_check_matches_oracle(fp, orc, workloads, min
     assert_particles_equal(got, want, f"range check z_min={min_height}")
 
 
+def test_step_in_a_set_beyond_32bit_offsets_matches_oracle(fp, orc, small):
+    """A particle set of more than 2^32 bytes (capacity above 44,739,242
+    slots) is addressed with 64-bit field bases, a smaller one with 32-bit
+    offsets from the set's base: the production step (4 sub-steps, one
+    particle per thread) in a context of each kind, bit for bit against the
+    oracle."""
+    w = small
+    parts = w.particles[:4096].copy()
+    met, pbl = met_dict(w.met), pbl_dict(w.pbl)
+    for cap in (len(parts), 44_739_243):
+        say(f"step parity: capacity {cap}")
+        want = parts.copy()
+        with fp.GpuTimestep(cap, w.met.shape, n_species=4) as ctx:
+            u0, v0, w0 = w.met.levels[0]
+            u1, v1, w1 = w.met.levels[1]
+            ctx.upload_met_bracket(u0, v0, w0, u1, v1, w1, w.met.level_heights)
+            ctx.upload_pbl(w.pbl.ustar, w.pbl.wstar, w.pbl.hmix, w.pbl.oli, w.pbl.precip)
+            ctx.upload_store(parts)
+            ctx.set_dispatch_count(len(parts))
+            ctx.set_launch(256, 1)
+            counter = (5, 0, 0, 0)
+            for _ in range(2):
+                cfg = base_cfg(fp, w, counter=counter, n_substeps=4, dry_mode=1, dry_vd=2e-3)
+                nxt = ctx.step(cfg)
+                onxt = orc.step(want, met, pbl, oracle_params(w, cfg))
+                assert list(nxt) == [int(v) for v in onxt]
+                counter = nxt
+            got = ctx.download_particles(len(parts))
+        assert_particles_equal(got, want, f"capacity {cap}")
+
+
 @pytest.mark.parametrize("case", ["production", "three_substeps"])
 @pytest.mark.parametrize("ppt", [1, 2, 4])
 @pytest.mark.parametrize("block", [64, 128, 256, 512, 1024])
