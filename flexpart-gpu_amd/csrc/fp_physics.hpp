@@ -115,7 +115,9 @@ __device__ __forceinline__ void gauss_pair(uint32_t a, uint32_t b, float &g0, fl
     // reference's upper clamps (and the lower one of au) leave it unchanged
     const float ru = fmaxf(unit24(a), 1.1754944e-38f);
     const float au = unit24(b);
-    const float rad = sqrt_cr_finite(-2.0f * fpm_logf_unit(ru));  // ru in [2^-126, 1): at most 175
+    // ru in [2^-126, 1 - 2^-24]: -2 log ru lies in [1.19e-7, 174.7] (every one
+    // of the 2^24 values checked on the CPU), inside sqrt_in_range's range
+    const float rad = sqrt_in_range(-2.0f * fpm_logf_unit(ru));
     float sn, cs;
     fpm_sincosf(6.2831855f * au, &sn, &cs);
     g0 = rad * cs;
@@ -320,6 +322,14 @@ __device__ __forceinline__ Dz tsqrt(Dz x)
     const float r = sqrt_in_range(x.v, &hy);
     return Dz{r, x.d * hy};
 }
+// (of an argument known to lie in [2^-96, FLT_MAX]: no range test)
+__device__ __forceinline__ float tsqrt_in_range(float x) { return sqrt_in_range(x); }
+__device__ __forceinline__ Dz tsqrt_in_range(Dz x)
+{
+    float hy;
+    const float r = sqrt_in_range(x.v, &hy);
+    return Dz{r, x.d * hy};
+}
 // (of an argument known not to be +infinity)
 __device__ __forceinline__ float tsqrt_finite(float x) { return sqrt_cr_finite(x); }
 __device__ __forceinline__ Dz tsqrt_finite(Dz x)
@@ -493,7 +503,8 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
         sigw = sraw + SIGMA_FLOOR;
     } else if (uns) {
         const T sq = 1.2f * wst * wst * (1.0f - 0.9f * zeta) * pa + (1.8f - 1.4f * zeta) * ust * ust;
-        sigw = tsqrt(sq) + SIGMA_FLOOR;
+        // under LEAN sq lies in [0.4 u*^2, 2^43] (u* >= 1e-4 and u*, w* <= 2^20)
+        sigw = (LEAN ? tsqrt_in_range(sq) : tsqrt(sq)) + SIGMA_FLOOR;
     } else {
         sigw = SIGMA_FLOOR + 1.3f * ust * (1.0f - zeta);
     }
@@ -604,7 +615,8 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
         tlu = d5;
         tlv = tlu;
     } else {
-        tlu = d5 * tsqrt_finite(zeta);  // zeta in [0, 1]
+        // zeta in [0, 1]; under LEAN zeta = z / h >= 2^-60
+        tlu = d5 * (LEAN ? tsqrt_in_range(zeta) : tsqrt_finite(zeta));
         tlv = 0.467f * tlu;
     }
     T sigw = v.sigw, dsigwdz = v.dsigwdz, tlw = v.tlw;
@@ -683,9 +695,11 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
 // are then the identity and false, and are not evaluated.
 
 // BR: the branch of every lane of the wavefront (0: r < 0.5, 1: r >= 0.5) or
-// -1 for a wavefront with both (selected per lane).  Under LEAN, r > 0 (dt
-// >= 2^-20, T_L >= 10) and e^{-r} lies in [0, 1], so the reference's clamps
-// of 2 r and 1 - e^{-2r} to [0, inf) are the identity.
+// -1 for a wavefront with both (selected per lane).  Under LEAN, r lies in
+// [2^-47, 2^41] (dt >= 2^-20, T_L in [10, 2^27]) and e^{-r} in [0, 1], so the
+// reference's clamps of 2 r and 1 - e^{-2r} to [0, inf) are the identity and
+// the root's argument lies in [2^-46, 1] (2 r >= 2^-46 where r < 0.5,
+// 1 - e^{-2r} >= 1 - e^{-1} where r >= 0.5): no range test.
 template <int BR, bool LEAN, class T>
 __device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta, float *coef)
 {
@@ -694,7 +708,7 @@ __device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta, fl
     T e = kon<T>(0.0f);
     if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
-    const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
+    const T root = LEAN ? tsqrt_in_range(arg) : tsqrt_finite(tmax(arg, 0.0f));
     const T c = small ? 1.0f - r : e;
     if (coef) *coef = val(c);
     const T nx = c * prev + eta * sigma * root;
@@ -710,7 +724,7 @@ __device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, f
     T e = kon<T>(0.0f);
     if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
-    const T root = tsqrt_finite(LEAN ? arg : tmax(arg, 0.0f));
+    const T root = LEAN ? tsqrt_in_range(arg) : tsqrt_finite(tmax(arg, 0.0f));
     const T c = small ? 1.0f - r : e;
     if (coef) *coef = val(c);
     const T nx = c * prev + eta * sw * root + (small ? dt * drift : tl * (1.0f - e) * drift);

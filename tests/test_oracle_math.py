@@ -80,3 +80,16 @@ def test_exp_log_special_values(orc):
     lg = orc.math_eval("log", np.array([1.0, 0.0, np.inf, 2.0], np.float32))
     assert lg[0] == 0.0 and lg[1] == -np.inf and lg[2] == np.inf
     assert lg[3] == np.float32(np.log(2.0))
+
+
+def test_box_muller_radius_argument_range(orc):
+    """Every Box-Muller radius argument -2 log u of the step, u the 24-bit
+    uniform of a Philox word floored at 2^-126 (langevin_fused.wgsl:186-192),
+    lies in [2^-96, FLT_MAX]: the range where the step's square root needs no
+    range test.  All 2^24 values of u."""
+    u = (np.arange(1 << 24, dtype=np.float64) / float(1 << 24)).astype(np.float32)
+    u = np.maximum(u, np.float32(1.1754944e-38))
+    arg = (np.float32(-2.0) * orc.math_eval("log", u)).astype(np.float32)
+    assert np.all(np.isfinite(arg))
+    assert float(arg.min()) >= 2.0 ** -96 and float(arg.min()) > 1.1e-7
+    assert float(arg.max()) <= 175.0
