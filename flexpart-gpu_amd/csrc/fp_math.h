@@ -337,9 +337,8 @@ FPM_FN double fpm_log_d(double x) { return fpm_log_d_tab(x, FPM_LOG_TAB); }
  * before the final rounding (overflow and underflow selected at the end).
  * t = (32 k' + j) ln2/32 + r, |r| <= ln2/64: e^t = 2^k' 2^(j/32) e^r with
  * e^r - 1 by its Taylor series to degree 6. */
-FPM_FN float fpm_exp_d_to_f_tab(double t, const double *exp2j)
+FPM_FN float fpm_exp_d_to_f_core(double tc, const double *exp2j)
 {
-    const double tc = __builtin_fmin(__builtin_fmax(t, -104.0), 89.0);
     const double k = __builtin_rint(tc * 46.16624130844683);
     /* ln2/32 = 0.021660849392446835 (exact, 16 trailing zero bits) + 5.1456e-14 */
     double r = __builtin_fma(k, -0.021660849392446835, tc);
@@ -354,10 +353,18 @@ FPM_FN float fpm_exp_d_to_f_tab(double t, const double *exp2j)
     const int j = ki & 31, ke = ki >> 5;
     const double y = __builtin_fma(exp2j[j], p, exp2j[j]);
     /* binary64 scaling is exact for ke in [-152, 130] */
-    float res = (float)(y * fpm_u2d((uint64_t)(ke + 1023) << 52));
+    return (float)(y * fpm_u2d((uint64_t)(ke + 1023) << 52));
+}
+FPM_FN float fpm_exp_d_to_f_tab(double t, const double *exp2j)
+{
+    const double tc = __builtin_fmin(__builtin_fmax(t, -104.0), 89.0);
+    float res = fpm_exp_d_to_f_core(tc, exp2j);
     res = t > 89.0 ? FPM_INF : res;
     return t < -104.0 ? 0.0f : res;
 }
+/* fpm_exp_d_to_f_tab of a t known to lie in [-104, 89], where the clamp and
+ * the overflow and underflow selects leave every value unchanged. */
+FPM_FN float fpm_exp_d_to_f_in_range(double t, const double *exp2j) { return fpm_exp_d_to_f_core(t, exp2j); }
 FPM_FN float fpm_exp_d_to_f(double t) { return fpm_exp_d_to_f_tab(t, FPM_EXP2_J); }
 
 /* x^y, single precision, for x >= 0 (negative x gives NaN). */

@@ -415,7 +415,26 @@ __device__ __forceinline__ float lds_powf(float x, float y)
     return fpm_powf_tab(x, y, math_tab(), math_tab() + 256);
 }
 
-// zeta^y as exp(y log zeta) in binary64 with log zeta = L precomputed
+// zeta^y as exp(y log zeta) in binary64 with log zeta = L precomputed; the
+// _in_range forms for y L known to lie in [-104, 89]
+__device__ __forceinline__ float pow_log_in_range(float, float y, double L)
+{
+    return fpm_exp_d_to_f_in_range((double)y * L, math_tab() + 256);
+}
+__device__ __forceinline__ float pow_log_floor_in_range(float, float y, bool use, double L, double Lfloor)
+{
+    return fpm_exp_d_to_f_in_range((double)y * (use ? L : Lfloor), math_tab() + 256);
+}
+__device__ __forceinline__ Dz pow_log_in_range(Dz zeta, float y, double L)
+{
+    const float v = fpm_exp_d_to_f_in_range((double)y * L, math_tab() + 256);
+    return Dz{v, zeta.v > 0.0f ? v * y / zeta.v * zeta.d : 0.0f};
+}
+__device__ __forceinline__ Dz pow_log_floor_in_range(Dz zeta, float y, bool use, double L, double Lfloor)
+{
+    const float v = fpm_exp_d_to_f_in_range((double)y * (use ? L : Lfloor), math_tab() + 256);
+    return Dz{v, (use && zeta.v > 0.0f) ? v * y / zeta.v * zeta.d : 0.0f};
+}
 __device__ __forceinline__ float pow_log(float, float y, double L) { return lds_exp_d_to_f((double)y * L); }
 __device__ __forceinline__ Dz pow_log(Dz zeta, float y, double L)
 {
@@ -492,7 +511,15 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
     T E = kon<T>(0.0f);
     if (any_nu) E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
     T pa = kon<T>(0.0f), pb = kon<T>(0.0f);
-    if (!neu) {
+    if (!neu && LEAN) {
+        // zeta in [2^-60, 1] (lean_lane: z >= 2^-40, h <= 2^20), so log zeta
+        // lies in [-41.6, 0], the exponents 0.8, 2/3 and -1/3 times it in
+        // [-33.3, 13.9] (log 1e-3 = -6.9 too): the zeta > 0 tests are true and
+        // the exponentials need neither clamp nor overflow/underflow selects
+        const double L = lds_log_d((double)val(zeta));
+        pa = pow_log_in_range(zeta, uns ? 2.0f / 3.0f : 0.8f, L);
+        if (uns) pb = pow_log_floor_in_range(zeta, -1.0f / 3.0f, val(zeta) >= 1.0e-3f, L, log_1em3);
+    } else if (!neu) {
         const double L = lds_log_d((double)(val(zeta) > 0.0f ? val(zeta) : 1.0f));
         pa = val(zeta) > 0.0f ? pow_log(zeta, uns ? 2.0f / 3.0f : 0.8f, L) : kon<T>(0.0f);
         if (uns) pb = pow_log_floor(zeta, -1.0f / 3.0f, val(zeta) >= 1.0e-3f, L, log_1em3);
