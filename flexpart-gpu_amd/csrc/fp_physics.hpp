@@ -465,9 +465,9 @@ struct TurbRegime {
     // range scaling (lean_lane)
     bool lean;
     // wavefront-uniform: with lean, the regime every active lane of the Hanna
-    // evaluation shares (0 neutral, 1 unstable, 2 stable), else -1; it holds
-    // for the later profile evaluations of the step, whose active lanes are
-    // the same
+    // evaluation shares (0 neutral, 1 unstable, 2 stable) or 3 for lanes in
+    // different regimes; -1 without lean.  It holds for the later profile
+    // evaluations of the step, whose active lanes are the same.
     int wr;
 };
 template <class T> struct TurbT : TurbRegime {
@@ -579,11 +579,11 @@ __device__ __forceinline__ void vertical_profile_r(const TurbRegime &t, T z_in, 
     }
 }
 
-// The regime-selected evaluation: a wavefront whose active lanes share one
-// regime and pass the range check (t.wr >= 0, decided once per step by
-// hanna_full) runs that regime's instantiation
-// with unscaled quotients (no per-lane selects between the regimes'
-// operands); any other wavefront runs the per-lane selected one with IEEE
+// The regime-selected evaluation: a wavefront whose active lanes pass the
+// range check (t.wr >= 0, decided once per step by hanna_full) runs with
+// unscaled quotients, in its regime's instantiation when its lanes share one
+// (no per-lane selects between the regimes' operands) or in the per-lane
+// selected one; any other wavefront runs the per-lane selected one with IEEE
 // quotients.  The arithmetic of each regime is the same in every
 // instantiation, and under t.lean both quotient forms give the same values.
 template <class T>
@@ -593,6 +593,7 @@ __device__ __forceinline__ void vertical_profile(const TurbRegime &t, T z_in, do
     if (t.wr == 0) return vertical_profile_r<0, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
     if (t.wr == 1) return vertical_profile_r<1, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
     if (t.wr == 2) return vertical_profile_r<2, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
+    if (t.wr == 3) return vertical_profile_r<-1, true>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
     vertical_profile_r<-1, false>(t, z_in, log_1em3, sigw, dsigwdz, tlw);
 }
 
@@ -699,11 +700,12 @@ __device__ __forceinline__ TurbT<T> hanna_full(float4 pb, float4 pc, T z_in, dou
     t.wr = -1;
     if (t.lean) {
         const uint64_t live = __ballot(1), n = __ballot(t.neu), u = __ballot(t.uns);
-        t.wr = n == live ? 0 : (u == live ? 1 : ((n | u) == 0ull ? 2 : -1));
+        t.wr = n == live ? 0 : (u == live ? 1 : ((n | u) == 0ull ? 2 : 3));
     }
     if (t.wr == 0) return hanna_full_r<0, true>(t, z_in, log_1em3);
     if (t.wr == 1) return hanna_full_r<1, true>(t, z_in, log_1em3);
     if (t.wr == 2) return hanna_full_r<2, true>(t, z_in, log_1em3);
+    if (t.wr == 3) return hanna_full_r<-1, true>(t, z_in, log_1em3);
     return hanna_full_r<-1, false>(t, z_in, log_1em3);
 }
 
