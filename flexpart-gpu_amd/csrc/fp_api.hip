@@ -175,7 +175,7 @@ void destroy(fp_ctx *ctx)
 // ---------------------------------------------------------------- shared helpers
 
 fp::GridArgs fp::make_grid_args(uint32_t *counts, uint32_t *mass_fp, uint32_t nx, uint32_t ny, uint32_t nz,
-                                const fp_grid_params &gp)
+                                const fp_grid_params &gp, uint32_t met_nx, uint32_t met_ny)
 {
     fp::GridArgs g{};
     g.counts = counts;
@@ -189,6 +189,11 @@ fp::GridArgs fp::make_grid_args(uint32_t *counts, uint32_t *mass_fp, uint32_t nx
     g.mass_scale = gp.mass_scale;
     g.metres = nz ? (gp.outheights[nz - 1] > 0.0f) : 0u;
     for (int k = 0; k < FP_MAX_OUTPUT_LEVELS; ++k) g.outheights[k] = gp.outheights[k];
+    bool sorted = true;
+    for (uint32_t k = 1; k < nz; ++k) sorted = sorted && gp.outheights[k] >= gp.outheights[k - 1];
+    const uint64_t lim = 0x7fffffffull;
+    g.fast = met_nx && met_ny && (uint64_t)met_nx * gp.refine_x <= lim && (uint64_t)met_ny * gp.refine_y <= lim &&
+             nz >= 1 && (!g.metres || sorted);
     return g;
 }
 
@@ -777,7 +782,7 @@ int fp_forcing_upload(fp_ctx *ctx, const float *dry_vd, const float *wet_lambda,
 static fp::GridArgs grid_args(const fp_ctx *ctx)
 {
     return fp::make_grid_args(ctx->grid_counts, ctx->grid_mass, ctx->desc.out_nx, ctx->desc.out_ny,
-                              ctx->desc.out_nz, ctx->grid_params);
+                              ctx->desc.out_nz, ctx->grid_params, ctx->desc.met_nx, ctx->desc.met_ny);
 }
 
 int fp_grid_configure(fp_ctx *ctx, const fp_grid_params *params)

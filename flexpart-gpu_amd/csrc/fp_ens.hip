@@ -481,7 +481,8 @@ int fp_ens_configure(fp_ens *e, const fp_step_params *p, const uint32_t *keys, c
         }
         if (e->grid_counts)
             a.grid = fp::make_grid_args(e->grid_counts + (uint64_t)m * e->ncell(),
-                                        e->grid_mass + (uint64_t)m * e->ncell(), e->ox, e->oy, e->oz, e->gp);
+                                        e->grid_mass + (uint64_t)m * e->ncell(), e->ox, e->oy, e->oz, e->gp,
+                                        e->nx, e->ny);
         a.ppt = e->ppt;
         for (int k = 0; k < 4; ++k) e->members[m].base_ctr[k] = counters ? counters[4 * m + k] : p->counter[k];
         e->members[m].ctr_per_step = fp::counter_per_step(n, p->n_substeps);

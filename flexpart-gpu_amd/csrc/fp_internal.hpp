@@ -171,6 +171,10 @@ struct GridArgs {
     uint32_t species;
     float mass_scale;
     uint32_t metres;  // outheights[nz-1] > 0
+    // the fused step's cells (met cells cx < met nx, cy < met ny) refine into
+    // int32 indices (met nx * refine_x and met ny * refine_y below 2^31) and
+    // the output heights, if used, are non-decreasing: grid_cell<true>
+    uint32_t fast;
     float outheights[FP_MAX_OUTPUT_LEVELS];
 };
 
@@ -464,7 +468,8 @@ uint32_t level_table(const float *lh, uint32_t nz, std::vector<float> &out);
 // deposition and loss modes) from the public parameter block.
 void fill_step_scalars(StepArgs &a, const fp_step_params &p);
 GridArgs make_grid_args(uint32_t *counts, uint32_t *mass_fp, uint32_t nx, uint32_t ny, uint32_t nz,
-                        const fp_grid_params &gp);
+                        const fp_grid_params &gp,
+                        uint32_t met_nx = 0, uint32_t met_ny = 0);
 // 128-bit little-endian counter + off (gpu/langevin_fused.rs:229-233).
 void counter_advance(const uint32_t c[4], uint64_t off, uint32_t out[4]);
 // Philox counter blocks one step consumes: dispatch * blocks per particle.

@@ -109,7 +109,7 @@ FPM_FN float fpm_div_unscaled_rcp(float n, float d, float *rcp)
 
 /* e^x, single precision.  Special cases are selected at the end rather than
  * branched on, so a wavefront never splits inside the function. */
-FPM_FN float fpm_expf(float x)
+FPM_FN float fpm_expf_core(float x)
 {
     const float xc = fpm_clampf(x, -104.0f, 89.0f);
     const float k = __builtin_rintf(xc * 1.44269502162933349609375f);
@@ -126,11 +126,17 @@ FPM_FN float fpm_expf(float x)
     const float p = __builtin_fmaf(q, r * r, r);
     /* ldexp rounds y * 2^k once, exactly like fpm_scale2; it overflows to
      * +inf for x in (88.72, 89]. */
-    const float res = __builtin_ldexpf(1.0f + p, (int)k);
+    return __builtin_ldexpf(1.0f + p, (int)k);
+}
+FPM_FN float fpm_expf(float x)
+{
     /* every x below -104 (and -inf) is evaluated at -104, whose result is 0
      * (e^-104 is below half the smallest subnormal); only NaN is selected */
+    const float res = fpm_expf_core(x);
     return x != x ? x : res;
 }
+/* fpm_expf of an x known not to be NaN (the select leaves it unchanged). */
+FPM_FN float fpm_expf_not_nan(float x) { return fpm_expf_core(x); }
 
 /* Natural logarithm, single precision (special cases selected, not branched). */
 FPM_FN float fpm_logf(float x)

@@ -309,6 +309,16 @@ __device__ __forceinline__ Dz texp(Dz x)
     const float e = fpm_expf(x.v);
     return Dz{e, e * x.d};
 }
+// texp of an argument that cannot be NaN when NOT_NAN holds
+template <bool NOT_NAN> __device__ __forceinline__ float texp_nn(float x)
+{
+    return NOT_NAN ? fpm_expf_not_nan(x) : fpm_expf(x);
+}
+template <bool NOT_NAN> __device__ __forceinline__ Dz texp_nn(Dz x)
+{
+    const float e = NOT_NAN ? fpm_expf_not_nan(x.v) : fpm_expf(x.v);
+    return Dz{e, e * x.d};
+}
 __device__ __forceinline__ float tsqrt(float x) { return sqrt_cr(x); }
 // (the derivative 1 / (2 sqrt(x)) from the root's own reciprocal square root
 // where the short sequence runs)
@@ -509,7 +519,8 @@ __device__ __forceinline__ VRawT<T> vertical_raw_r(const TurbRegime &t, T z, T z
     // same two regimes, so a wavefront of stable particles skips both
     const bool any_nu = R < 0 ? __ballot(neu || uns) != 0ull : R != 2;
     T E = kon<T>(0.0f);
-    if (any_nu) E = texp(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
+    // (under LEAN d0 and zeta are finite)
+    if (any_nu) E = texp_nn<LEAN>(neu ? -2.0e-4f * v.d0 : -5.0f * zeta);
     T pa = kon<T>(0.0f), pb = kon<T>(0.0f);
     if (!neu && LEAN) {
         // zeta in [2^-60, 1] (lean_lane: z >= 2^-40, h <= 2^20), so log zeta
@@ -624,7 +635,7 @@ __device__ __forceinline__ TurbT<T> hanna_full_r(TurbT<T> t, T z_in, double log_
     const VRawT<T> v = vertical_raw_r<R, LEAN>(t, z, zeta, log_1em3);
     T sigu, sigv, tlu, tlv;
     if (neu) {
-        sigu = SIGMA_FLOOR + 2.0f * ust * texp(-3.0e-4f * v.d0);
+        sigu = SIGMA_FLOOR + 2.0f * ust * texp_nn<LEAN>(-3.0e-4f * v.d0);
         sigv = v.sigw;
     } else if (uns) {
         sigu = kon<T>(t.sigu_cell);
@@ -735,7 +746,7 @@ __device__ __forceinline__ T ou_horizontal_b(T prev, T sigma, T r, float eta, fl
     const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
     // e^{-r} serves the r >= 0.5 branch only: skipped by a wavefront without one
     T e = kon<T>(0.0f);
-    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
+    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp_nn<LEAN>(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
     const T root = LEAN ? tsqrt_in_range(arg) : tsqrt_finite(tmax(arg, 0.0f));
     const T c = small ? 1.0f - r : e;
@@ -751,7 +762,7 @@ __device__ __forceinline__ T ou_vertical_b(T prev, T sw, T dsigwdz, T tl, T r, f
     const T drift = sw * (dsigwdz + rho * sw);
     const bool small = BR < 0 ? val(r) < 0.5f : BR == 0;
     T e = kon<T>(0.0f);
-    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp(-r);
+    if (BR < 0 ? __ballot(!small) != 0ull : BR == 1) e = texp_nn<LEAN>(-r);
     const T arg = small ? 2.0f * r : 1.0f - e * e;
     const T root = LEAN ? tsqrt_in_range(arg) : tsqrt_finite(tmax(arg, 0.0f));
     const T c = small ? 1.0f - r : e;
@@ -815,6 +826,13 @@ __device__ __forceinline__ void reflect(float &pz, float &tw, int32_t &cbt, floa
 
 // ---------------------------------------------------------------- gridding
 
+// FAST (g.fast, for the fused step's cells: cx and cy index the met grid):
+// cx * refine_x + fx and cy * refine_y + fy are non-negative and below 2^31,
+// so 32-bit arithmetic gives the same cell and only the upper clamp can act;
+// with non-decreasing output heights the levels below z are a prefix, and the
+// first k with z <= h[k] is their count (nz - 1 when there is none, NaN z
+// included).
+template <bool FAST = false>
 __device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, const float *oh, int32_t cx, int32_t cy,
                                               float px, float py, float z)
 {
@@ -822,6 +840,20 @@ __device__ __forceinline__ uint32_t grid_cell(const GridArgs &g, const float *oh
     int32_t fy = (int32_t)floorf(py * (float)g.refine_y);
     fx = max(0, min(fx, (int32_t)g.refine_x - 1));
     fy = max(0, min(fy, (int32_t)g.refine_y - 1));
+    if (FAST) {
+        const uint32_t ix = min((uint32_t)cx * g.refine_x + (uint32_t)fx, g.nx - 1u);
+        const uint32_t iy = min((uint32_t)cy * g.refine_y + (uint32_t)fy, g.ny - 1u);
+        uint32_t iz;
+        if (g.metres) {
+            uint32_t below = 0;
+            for (uint32_t k = 0; k < g.nz; ++k) below += !(z <= oh[k]) ? 1u : 0u;
+            iz = min(below, g.nz - 1u);
+        } else {
+            const int32_t l = (int32_t)floorf(z);
+            iz = (uint32_t)max(0, min(l, (int32_t)g.nz - 1));
+        }
+        return (ix * g.ny + iy) * g.nz + iz;
+    }
     int64_t ix = (int64_t)cx * g.refine_x + fx;
     int64_t iy = (int64_t)cy * g.refine_y + fy;
     ix = ix < 0 ? 0 : (ix > (int64_t)g.nx - 1 ? (int64_t)g.nx - 1 : ix);

@@ -204,7 +204,8 @@ __device__ __forceinline__ void step_chunk(const StepArgs &a, uint64_t first, fl
 
             if (a.grid_fused && (flags & FLAG_ACTIVE)) {
                 grid_valid = true;
-                grid_c = grid_cell(a.grid, oh, cx, cy, px, py, pz);
+                grid_c = a.grid.fast ? grid_cell<true>(a.grid, oh, cx, cy, px, py, pz)
+                                     : grid_cell(a.grid, oh, cx, cy, px, py, pz);
                 grid_q = quantize_mass(mass[a.grid.species], a.grid.mass_scale);
             }
         }
