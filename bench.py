@@ -274,23 +274,24 @@ def main(argv=None) -> int:
     achieved_wall = n * BYTES_PER_PARTICLE_STEP / (elapsed / args.steps) / 1e9
     traffic = _traffic_bytes()
     valu_row = _profile_figure(VALU_FILE, "valu_per_row")
+    insts_row = _profile_figure(VALU_FILE, "insts_per_row")
     fast_ns = _profile_figure(RATES_FILE, "fast_form_ns")
     slow_ns = _profile_figure(RATES_FILE, "slow_form_ns")
     valu = None
-    if valu_row and fast_ns and slow_ns:
-        per_simd = valu_row * (n / 64.0) / SIMDS  # wavefront VALU instructions per SIMD per launch
+    if valu_row and insts_row and fast_ns and slow_ns:
+        rows_per_simd = (n / 64.0) / SIMDS
         valu = {"valu_per_64_particle_row": valu_row,
-                "ns_per_valu_per_simd": avg_s * 1e9 / per_simd,
+                "insts_per_64_particle_row": insts_row,
+                "ns_per_inst_per_simd": avg_s * 1e9 / (insts_row * rows_per_simd),
                 "fast_form_ns": fast_ns, "slow_form_ns": slow_ns,
-                "frac_at_fast_rate": per_simd * fast_ns * 1e-9 / avg_s,
-                "frac_at_slow_rate": per_simd * slow_ns * 1e-9 / avg_s,
-                "note": "SQ_INSTS_VALU per 64-particle row (committed rocprofv3 summary of this workload) "
-                        "issued on 1024 SIMDs: the event-timed launch spends ns_per_valu_per_simd per "
-                        "instruction; the measured issue cost of single instruction forms on this GPU is "
-                        "fast_form_ns (f32 add/mul/fmaak on vector registers) to slow_form_ns (scalar-register "
-                        "operands, compares, selects, min/max, shifts, conversions, f64 and 64-bit integer "
-                        "forms), so all of the kernel's instructions at the fast rate would fill "
-                        "frac_at_fast_rate of the launch and all at the slow rate frac_at_slow_rate"}
+                "frac_at_issue_rate": insts_row * rows_per_simd * fast_ns * 1e-9 / avg_s,
+                "note": "SQ_INSTS (all issued instructions, vector and scalar) and SQ_INSTS_VALU per "
+                        "64-particle row, from the committed rocprofv3 summary of this workload, on 1024 "
+                        "SIMDs: the event-timed launch spends ns_per_inst_per_simd per issued instruction; "
+                        "the measured issue rate of this GPU is one instruction per fast_form_ns per SIMD "
+                        "(slow_form_ns for like slow forms back to back, which interleaved issue at the "
+                        "fast rate), so frac_at_issue_rate is the share of the launch its instructions "
+                        "need at that rate"}
     result = {
         "metric": METRIC,
         "value": n * args.steps / elapsed,
