@@ -286,7 +286,8 @@ def main(argv=None) -> int:
                 "fast_form_ns": fast_ns, "slow_form_ns": slow_ns,
                 "frac_at_issue_rate": insts_row * rows_per_simd * fast_ns * 1e-9 / avg_s,
                 "note": "SQ_INSTS (all issued instructions, vector and scalar) and SQ_INSTS_VALU per "
-                        "64-particle row, from the committed rocprofv3 summary of this workload, on 1024 "
+                        "64-particle row, from the committed rocprofv3 summary of this workload (means over "
+                        "the 24 h window; the first steps after a reset issue a few per cent more), on 1024 "
                         "SIMDs: the event-timed launch spends ns_per_inst_per_simd per issued instruction; "
                         "the measured issue rate of this GPU is one instruction per fast_form_ns per SIMD "
                         "(slow_form_ns for like slow forms back to back, which interleaved issue at the "
